@@ -1,0 +1,205 @@
+// Short-Weierstrass (a = 0) point arithmetic over Fq (G1) and Fq2 (G2) for gfx950.
+//
+// The MSM works in extended Jacobian "XYZZ" coordinates (x = X/ZZ, y = Y/ZZZ, ZZ^3 = ZZZ^2;
+// infinity = ZZ = 0), which make the mixed add 8M + 2S and the full add 12M + 2S with no
+// Z^3 products (EFD "add-2008-s", "madd-2008-s", "dbl-2008-s-1", "mdbl-2008-s-1").
+// Affine inputs use the reference encoding: infinity = (0, 0) (bend2/FORMAT.md:12-13).
+// Every special case (infinity operand, P = Q, P = -Q) is resolved exactly, so any
+// accumulation order yields the same group element and hence the same affine bits.
+//
+// JacRef restates the reference's Jacobian formulas (bend2/gen/gen_curve.py:86-184:
+// dbl-2009-l, madd-2007-bl) for the on-device known-answer chain only.
+#pragma once
+#include "bg_field.hpp"
+
+namespace bg {
+
+template <class F>
+struct Aff {
+  typename F::E x, y;
+};
+
+template <class F>
+struct Xyzz {
+  typename F::E X, Y, ZZ, ZZZ;
+};
+
+template <class F>
+struct Curve {
+  using E = typename F::E;
+  using A = Aff<F>;
+  using P = Xyzz<F>;
+
+  __device__ __forceinline__ static P infinity() { return {F::zero(), F::zero(), F::zero(), F::zero()}; }
+  __device__ __forceinline__ static bool is_inf(const P& p) { return F::is_zero(p.ZZ); }
+  __device__ __forceinline__ static bool aff_is_inf(const A& a) { return F::is_zero(a.x) && F::is_zero(a.y); }
+  __device__ __forceinline__ static P from_affine(const A& a) {
+    if (aff_is_inf(a)) return infinity();
+    return {a.x, a.y, F::one(), F::one()};
+  }
+  __device__ __forceinline__ static A aff_neg(const A& a) { return {a.x, F::neg(a.y)}; }
+  __device__ __forceinline__ static P neg(const P& p) { return {p.X, F::neg(p.Y), p.ZZ, p.ZZZ}; }
+
+  // 2 * affine (a, not infinity): mdbl-2008-s-1
+  __device__ static P dbl_aff(const A& a) {
+    E u = F::dbl(a.y);
+    E v = F::sqr(u);
+    E w = F::mul(u, v);
+    E s = F::mul(a.x, v);
+    E x2 = F::sqr(a.x);
+    E m = F::add(F::dbl(x2), x2);
+    E x3 = F::sub(F::sqr(m), F::dbl(s));
+    E y3 = F::sub(F::mul(m, F::sub(s, x3)), F::mul(w, a.y));
+    return {x3, y3, v, w};
+  }
+
+  // 2 * p: dbl-2008-s-1 (infinity stays infinity since V*ZZ1 = 0)
+  __device__ static P dbl(const P& p) {
+    E u = F::dbl(p.Y);
+    E v = F::sqr(u);
+    E w = F::mul(u, v);
+    E s = F::mul(p.X, v);
+    E x2 = F::sqr(p.X);
+    E m = F::add(F::dbl(x2), x2);
+    E x3 = F::sub(F::sqr(m), F::dbl(s));
+    E y3 = F::sub(F::mul(m, F::sub(s, x3)), F::mul(w, p.Y));
+    return {x3, y3, F::mul(v, p.ZZ), F::mul(w, p.ZZZ)};
+  }
+
+  // p + a (mixed): madd-2008-s with exact special cases
+  __device__ static P madd(const P& p, const A& a) {
+    if (aff_is_inf(a)) return p;
+    if (is_inf(p)) return {a.x, a.y, F::one(), F::one()};
+    E u2 = F::mul(a.x, p.ZZ);
+    E s2 = F::mul(a.y, p.ZZZ);
+    E pp_ = F::sub(u2, p.X);
+    E r = F::sub(s2, p.Y);
+    if (F::is_zero(pp_)) {
+      if (F::is_zero(r)) return dbl_aff(a);
+      return infinity();
+    }
+    E pp = F::sqr(pp_);
+    E ppp = F::mul(pp_, pp);
+    E q = F::mul(p.X, pp);
+    E x3 = F::sub(F::sub(F::sqr(r), ppp), F::dbl(q));
+    E y3 = F::sub(F::mul(r, F::sub(q, x3)), F::mul(p.Y, ppp));
+    return {x3, y3, F::mul(p.ZZ, pp), F::mul(p.ZZZ, ppp)};
+  }
+
+  // p + q: add-2008-s with exact special cases
+  __device__ static P add(const P& p, const P& q) {
+    if (is_inf(p)) return q;
+    if (is_inf(q)) return p;
+    E u1 = F::mul(p.X, q.ZZ);
+    E u2 = F::mul(q.X, p.ZZ);
+    E s1 = F::mul(p.Y, q.ZZZ);
+    E s2 = F::mul(q.Y, p.ZZZ);
+    E pp_ = F::sub(u2, u1);
+    E r = F::sub(s2, s1);
+    if (F::is_zero(pp_)) {
+      if (F::is_zero(r)) return dbl(p);
+      return infinity();
+    }
+    E pp = F::sqr(pp_);
+    E ppp = F::mul(pp_, pp);
+    E q_ = F::mul(u1, pp);
+    E x3 = F::sub(F::sub(F::sqr(r), ppp), F::dbl(q_));
+    E y3 = F::sub(F::mul(r, F::sub(q_, x3)), F::mul(s1, ppp));
+    return {x3, y3, F::mul(F::mul(p.ZZ, q.ZZ), pp), F::mul(F::mul(p.ZZZ, q.ZZZ), ppp)};
+  }
+
+  // k doublings
+  __device__ static P dbl_n(P p, int k) {
+    for (int i = 0; i < k; i++) p = dbl(p);
+    return p;
+  }
+
+  // XYZZ -> affine with one inversion: t = 1/(ZZ*ZZZ); x = X*ZZZ*t; y = Y*ZZ*t
+  __device__ __noinline__ static A to_affine(const P& p) {
+    if (is_inf(p)) return {F::zero(), F::zero()};
+    E t = F::inv(F::mul(p.ZZ, p.ZZZ));
+    return {F::mul(F::mul(p.X, p.ZZZ), t), F::mul(F::mul(p.Y, p.ZZ), t)};
+  }
+
+  // memory layout of an XYZZ point: X, Y, ZZ, ZZZ (4 elements)
+  static constexpr int elem_words = Ld<E>::words;
+  static constexpr int xyzz_words = 4 * elem_words;
+  static constexpr int aff_words = 2 * elem_words;
+
+  __device__ __forceinline__ static P load(const uint32_t* q) {
+    return {Ld<E>::load(q), Ld<E>::load(q + elem_words), Ld<E>::load(q + 2 * elem_words),
+            Ld<E>::load(q + 3 * elem_words)};
+  }
+  __device__ __forceinline__ static void store(uint32_t* q, const P& p) {
+    Ld<E>::store(q, p.X);
+    Ld<E>::store(q + elem_words, p.Y);
+    Ld<E>::store(q + 2 * elem_words, p.ZZ);
+    Ld<E>::store(q + 3 * elem_words, p.ZZZ);
+  }
+  __device__ __forceinline__ static A load_aff(const uint32_t* q) {
+    return {Ld<E>::load(q), Ld<E>::load(q + elem_words)};
+  }
+  __device__ __forceinline__ static void store_aff(uint32_t* q, const A& a) {
+    Ld<E>::store(q, a.x);
+    Ld<E>::store(q + elem_words, a.y);
+  }
+};
+
+using G1 = Curve<Fq>;
+using G2 = Curve<Fq2>;
+
+// ---------------------------------------------------------------- reference Jacobian
+// gen_curve.py:86-102 (dbl-2009-l) and :147-184 (madd-2007-bl + special cases);
+// infinity = (0, 1, 0).  Used only by the known-answer chain kernel.
+template <class F>
+struct JacRef {
+  using E = typename F::E;
+  struct J {
+    E x, y, z;
+  };
+  __device__ static J infinity() { return {F::zero(), F::one(), F::zero()}; }
+  __device__ static J dbl(const J& p) {
+    E a = F::sqr(p.x);
+    E b = F::sqr(p.y);
+    E c = F::sqr(b);
+    E d0 = F::sub(F::sub(F::sqr(F::add(p.x, b)), a), c);
+    E d = F::add(d0, d0);
+    E e = F::add(F::add(a, a), a);
+    E f = F::sqr(e);
+    E x3 = F::sub(f, F::add(d, d));
+    E c2 = F::add(c, c);
+    E c4 = F::add(c2, c2);
+    E y3 = F::sub(F::mul(e, F::sub(d, x3)), F::add(c4, c4));
+    E z3 = F::mul(F::add(p.y, p.y), p.z);
+    return {x3, y3, z3};
+  }
+  __device__ static J madd(const J& p, const Aff<F>& q) {
+    if (F::is_zero(p.z)) {
+      if (F::is_zero(q.x) && F::is_zero(q.y)) return infinity();
+      return {q.x, q.y, F::one()};
+    }
+    if (F::is_zero(q.x) && F::is_zero(q.y)) return p;
+    E z1z1 = F::sqr(p.z);
+    E u2 = F::mul(q.x, z1z1);
+    E s2 = F::mul(F::mul(q.y, p.z), z1z1);
+    E h = F::sub(u2, p.x);
+    E rr = F::sub(s2, p.y);
+    if (F::is_zero(h)) {
+      if (F::is_zero(rr)) return dbl(p);
+      return infinity();
+    }
+    E hh = F::sqr(h);
+    E hh2 = F::add(hh, hh);
+    E i = F::add(hh2, hh2);
+    E j = F::mul(h, i);
+    E r = F::add(rr, rr);
+    E v = F::mul(p.x, i);
+    E x3 = F::sub(F::sub(F::sqr(r), j), F::add(v, v));
+    E y1j = F::mul(p.y, j);
+    E y3 = F::sub(F::mul(r, F::sub(v, x3)), F::add(y1j, y1j));
+    E z3 = F::sub(F::sub(F::sqr(F::add(p.z, h)), z1z1), hh);
+    return {x3, y3, z3};
+  }
+};
+
+}  // namespace bg

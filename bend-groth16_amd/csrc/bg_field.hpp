@@ -1,0 +1,301 @@
+// BN254 prime-field arithmetic for gfx950: 8 x 32-bit little-endian limbs held in VGPRs,
+// Montgomery form with R = 2^256 (the same integers as the reference's 16 x 16-bit limbs,
+// bend2/FORMAT.md:7-8, so repacking is a pure bit pack).
+//
+// The Montgomery product uses finely integrated product scanning: every column of
+// a*b + m*p is accumulated in a 96-bit (v_mad_u64_u32 + carry word) accumulator, the
+// reduction digit m_k = lo(acc) * n0 is formed in the same column, and the column is
+// shifted out.  136 32x32->64 multiplies per product (64 a*b, 64 m*p, 8 m digits).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bg {
+
+struct Fp {
+  uint32_t v[8];
+};
+
+// Field constants.  p < 2^254 for both BN254 fields, so a Montgomery product of inputs < p
+// is < 2p before the final conditional subtraction and never needs a 9th word.
+struct FrP {
+  static constexpr uint32_t p[8] = {0xf0000001u, 0x43e1f593u, 0x79b97091u, 0x2833e848u,
+                                    0x8181585du, 0xb85045b6u, 0xe131a029u, 0x30644e72u};
+  static constexpr uint32_t n0 = 0xefffffffu;
+  static constexpr uint32_t one[8] = {0x4ffffffbu, 0xac96341cu, 0x9f60cd29u, 0x36fc7695u,
+                                      0x7879462eu, 0x666ea36fu, 0x9a07df2fu, 0x0e0a77c1u};
+  static constexpr uint32_t r2[8] = {0xae216da7u, 0x1bb8e645u, 0xe35c59e3u, 0x53fe3ab1u,
+                                     0x53bb8085u, 0x8c49833du, 0x7f4e44a5u, 0x0216d0b1u};
+};
+struct FqP {
+  static constexpr uint32_t p[8] = {0xd87cfd47u, 0x3c208c16u, 0x6871ca8du, 0x97816a91u,
+                                    0x8181585du, 0xb85045b6u, 0xe131a029u, 0x30644e72u};
+  static constexpr uint32_t n0 = 0xe4866389u;
+  static constexpr uint32_t one[8] = {0xc58f0d9du, 0xd35d438du, 0xf5c70b3du, 0x0a78eb28u,
+                                      0x7879462cu, 0x666ea36fu, 0x9a07df2fu, 0x0e0a77c1u};
+  static constexpr uint32_t r2[8] = {0x538afa89u, 0xf32cfc5bu, 0xd44501fbu, 0xb5e71911u,
+                                     0x0a417ff6u, 0x47ab1effu, 0xcab8351fu, 0x06d89f71u};
+};
+
+// ------------------------------------------------------------------ carry helpers
+
+// 96-bit column accumulator: lo (64 bits) + hi (32 bits).
+struct Acc {
+  uint64_t lo;
+  uint32_t hi;
+};
+
+// acc += a * b: one v_mad_u64_u32 (64-bit accumulate, carry out to an SGPR pair) and one
+// v_addc_co_u32 folding that carry into the high word.  ALU-only inline assembly: no
+// memory operand, so no load can be left in flight.
+__device__ __forceinline__ void acc_mad(Acc& acc, uint32_t a, uint32_t b) {
+  uint64_t cc;
+  asm("v_mad_u64_u32 %0, %1, %3, %4, %0\n\t"
+      "v_addc_co_u32_e64 %2, %1, %2, 0, %1"
+      : "+v"(acc.lo), "=&s"(cc), "+v"(acc.hi)
+      : "v"(a), "v"(b));
+}
+// same with the second factor a wave-uniform constant held in an SGPR
+__device__ __forceinline__ void acc_mad_s(Acc& acc, uint32_t a, uint32_t k) {
+  uint64_t cc;
+  asm("v_mad_u64_u32 %0, %1, %3, %4, %0\n\t"
+      "v_addc_co_u32_e64 %2, %1, %2, 0, %1"
+      : "+v"(acc.lo), "=&s"(cc), "+v"(acc.hi)
+      : "v"(a), "s"(k));
+}
+
+__device__ __forceinline__ void acc_shift(Acc& acc) {
+  acc.lo = (acc.lo >> 32) | ((uint64_t)acc.hi << 32);
+  acc.hi = 0;
+}
+
+// r = a - p if a >= p else a (a < 2p)
+template <class P>
+__device__ __forceinline__ void reduce_once(uint32_t* r, const uint32_t* a) {
+  uint32_t d[8];
+  uint32_t br = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    uint64_t t = (uint64_t)a[i] - P::p[i] - br;
+    d[i] = (uint32_t)t;
+    br = (uint32_t)(t >> 63);
+  }
+  // br == 1 -> a < p: keep a
+#pragma unroll
+  for (int i = 0; i < 8; i++) r[i] = br ? a[i] : d[i];
+}
+
+// ------------------------------------------------------------------ field ops
+
+template <class P>
+struct Field {
+  using E = Fp;
+
+  __device__ __forceinline__ static Fp zero() {
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) r.v[i] = 0;
+    return r;
+  }
+  __device__ __forceinline__ static Fp one() {
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) r.v[i] = P::one[i];
+    return r;
+  }
+  __device__ __forceinline__ static bool is_zero(const Fp& a) {
+    uint32_t x = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) x |= a.v[i];
+    return x == 0;
+  }
+  __device__ __forceinline__ static bool eq(const Fp& a, const Fp& b) {
+    uint32_t x = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) x |= a.v[i] ^ b.v[i];
+    return x == 0;
+  }
+
+  __device__ __forceinline__ static Fp add(const Fp& a, const Fp& b) {
+    uint32_t s[8];
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      uint64_t t = (uint64_t)a.v[i] + b.v[i] + c;
+      s[i] = (uint32_t)t;
+      c = (uint32_t)(t >> 32);
+    }
+    Fp r;
+    reduce_once<P>(r.v, s);  // a + b < 2p < 2^255: no carry out of word 7
+    return r;
+  }
+
+  __device__ __forceinline__ static Fp sub(const Fp& a, const Fp& b) {
+    uint32_t d[8];
+    uint32_t br = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      uint64_t t = (uint64_t)a.v[i] - b.v[i] - br;
+      d[i] = (uint32_t)t;
+      br = (uint32_t)(t >> 63);
+    }
+    // borrow -> add p back
+    uint32_t mask = 0u - br;
+    Fp r;
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      uint64_t t = (uint64_t)d[i] + (P::p[i] & mask) + c;
+      r.v[i] = (uint32_t)t;
+      c = (uint32_t)(t >> 32);
+    }
+    return r;
+  }
+
+  __device__ __forceinline__ static Fp neg(const Fp& a) { return sub(zero(), a); }
+
+  __device__ __forceinline__ static Fp dbl(const Fp& a) { return add(a, a); }
+
+  // Montgomery product a * b * 2^-256 mod p, finely integrated product scanning.
+  __device__ __forceinline__ static Fp mul(const Fp& a, const Fp& b) {
+    uint32_t m[8];
+    uint32_t r[8];
+    Acc acc = {0, 0};
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+#pragma unroll
+      for (int i = 0; i < k; i++) {
+        acc_mad(acc, a.v[i], b.v[k - i]);
+        acc_mad_s(acc, m[i], P::p[k - i]);
+      }
+      acc_mad(acc, a.v[k], b.v[0]);
+      m[k] = (uint32_t)acc.lo * P::n0;
+      acc_mad_s(acc, m[k], P::p[0]);
+      acc_shift(acc);
+    }
+#pragma unroll
+    for (int k = 8; k < 15; k++) {
+#pragma unroll
+      for (int i = k - 7; i < 8; i++) {
+        acc_mad(acc, a.v[i], b.v[k - i]);
+        acc_mad_s(acc, m[i], P::p[k - i]);
+      }
+      r[k - 8] = (uint32_t)acc.lo;
+      acc_shift(acc);
+    }
+    r[7] = (uint32_t)acc.lo;
+    Fp o;
+    reduce_once<P>(o.v, r);
+    return o;
+  }
+
+  __device__ __forceinline__ static Fp sqr(const Fp& a) { return mul(a, a); }
+
+  // x^e for an exponent given as 8 little-endian words (left to right)
+  __device__ __noinline__ static Fp pow(const Fp& x, const uint32_t* e) {
+    Fp acc = one();
+    for (int i = 255; i >= 0; i--) {
+      acc = sqr(acc);
+      if ((e[i >> 5] >> (i & 31)) & 1) acc = mul(acc, x);
+    }
+    return acc;
+  }
+
+  // Fermat inverse a^(p-2); inv(0) = 0
+  __device__ static Fp inv(const Fp& a) {
+    uint32_t e[8];
+    uint32_t br = 2;
+    for (int i = 0; i < 8; i++) {
+      uint64_t t = (uint64_t)P::p[i] - br;
+      e[i] = (uint32_t)t;
+      br = (uint32_t)(t >> 63);
+    }
+    return pow(a, e);
+  }
+
+  // canonical -> Montgomery (x * R^2 * R^-1)
+  __device__ __forceinline__ static Fp to_mont(const Fp& a) {
+    Fp r2;
+#pragma unroll
+    for (int i = 0; i < 8; i++) r2.v[i] = P::r2[i];
+    return mul(a, r2);
+  }
+  // Montgomery -> canonical (x * 1 * R^-1)
+  __device__ __forceinline__ static Fp from_mont(const Fp& a) {
+    Fp o = zero();
+    o.v[0] = 1;
+    return mul(a, o);
+  }
+};
+
+using Fr = Field<FrP>;
+using Fq = Field<FqP>;
+
+// ------------------------------------------------------------------ Fq2 = Fq[u]/(u^2 + 1)
+
+struct Fp2 {
+  Fp c0, c1;
+};
+
+struct Fq2 {
+  using E = Fp2;
+  __device__ __forceinline__ static Fp2 zero() { return {Fq::zero(), Fq::zero()}; }
+  __device__ __forceinline__ static Fp2 one() { return {Fq::one(), Fq::zero()}; }
+  __device__ __forceinline__ static bool is_zero(const Fp2& a) { return Fq::is_zero(a.c0) && Fq::is_zero(a.c1); }
+  __device__ __forceinline__ static bool eq(const Fp2& a, const Fp2& b) { return Fq::eq(a.c0, b.c0) && Fq::eq(a.c1, b.c1); }
+  __device__ __forceinline__ static Fp2 add(const Fp2& a, const Fp2& b) { return {Fq::add(a.c0, b.c0), Fq::add(a.c1, b.c1)}; }
+  __device__ __forceinline__ static Fp2 sub(const Fp2& a, const Fp2& b) { return {Fq::sub(a.c0, b.c0), Fq::sub(a.c1, b.c1)}; }
+  __device__ __forceinline__ static Fp2 neg(const Fp2& a) { return {Fq::neg(a.c0), Fq::neg(a.c1)}; }
+  __device__ __forceinline__ static Fp2 dbl(const Fp2& a) { return {Fq::dbl(a.c0), Fq::dbl(a.c1)}; }
+  // Karatsuba: 3 Fq products
+  __device__ __forceinline__ static Fp2 mul(const Fp2& a, const Fp2& b) {
+    Fp t0 = Fq::mul(a.c0, b.c0);
+    Fp t1 = Fq::mul(a.c1, b.c1);
+    Fp t2 = Fq::mul(Fq::add(a.c0, a.c1), Fq::add(b.c0, b.c1));
+    return {Fq::sub(t0, t1), Fq::sub(Fq::sub(t2, t0), t1)};
+  }
+  // complex squaring: 2 Fq products
+  __device__ __forceinline__ static Fp2 sqr(const Fp2& a) {
+    return {Fq::mul(Fq::add(a.c0, a.c1), Fq::sub(a.c0, a.c1)), Fq::mul(Fq::dbl(a.c0), a.c1)};
+  }
+  __device__ __noinline__ static Fp2 inv(const Fp2& a) {
+    Fp n = Fq::add(Fq::sqr(a.c0), Fq::sqr(a.c1));
+    Fp ni = Fq::inv(n);
+    return {Fq::mul(a.c0, ni), Fq::neg(Fq::mul(a.c1, ni))};
+  }
+};
+
+// ------------------------------------------------------------------ loads / stores
+
+__device__ __forceinline__ Fp load_fp(const uint32_t* __restrict__ p) {
+  const uint4* q = reinterpret_cast<const uint4*>(p);
+  uint4 x = q[0], y = q[1];
+  return Fp{{x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w}};
+}
+__device__ __forceinline__ void store_fp(uint32_t* __restrict__ p, const Fp& a) {
+  uint4* q = reinterpret_cast<uint4*>(p);
+  q[0] = make_uint4(a.v[0], a.v[1], a.v[2], a.v[3]);
+  q[1] = make_uint4(a.v[4], a.v[5], a.v[6], a.v[7]);
+}
+__device__ __forceinline__ Fp2 load_fp2(const uint32_t* __restrict__ p) { return {load_fp(p), load_fp(p + 8)}; }
+__device__ __forceinline__ void store_fp2(uint32_t* __restrict__ p, const Fp2& a) {
+  store_fp(p, a.c0);
+  store_fp(p + 8, a.c1);
+}
+
+template <class E>
+struct Ld;
+template <>
+struct Ld<Fp> {
+  static constexpr int words = 8;
+  __device__ __forceinline__ static Fp load(const uint32_t* p) { return load_fp(p); }
+  __device__ __forceinline__ static void store(uint32_t* p, const Fp& a) { store_fp(p, a); }
+};
+template <>
+struct Ld<Fp2> {
+  static constexpr int words = 16;
+  __device__ __forceinline__ static Fp2 load(const uint32_t* p) { return load_fp2(p); }
+  __device__ __forceinline__ static void store(uint32_t* p, const Fp2& a) { store_fp2(p, a); }
+};
+
+}  // namespace bg

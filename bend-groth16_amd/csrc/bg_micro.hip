@@ -1,0 +1,288 @@
+// Known-answer kernels, element-wise field/curve kernels for the parity tests, and
+// throughput probes (C ABI: bg_kat_field_mul_xor, bg_kat_g1_add_chain, bg_field_op,
+// bg_curve_op, bg_probe_rate).
+#include "bg_curve.hpp"
+#include "bg_runtime.hpp"
+#include <string.h>
+#include <chrono>
+
+namespace bg {
+
+// ---------------------------------------------------------------- field_mul KAT
+// bend2/bench/field_mul.bend:9-31: h(i) = ((i * 2654435761) ^ (i >> 15)) * 2246822519,
+// operand s has 16-bit limbs h(s + off_k) (off_0 = 0, off_k = 40503 k + 1), top limb
+// masked to 0x2FFF; product i multiplies operands 2i and 2i+1; checksum = xor of the
+// low 16 bits of every Montgomery product.
+__device__ __forceinline__ uint32_t kat_h(uint32_t i) { return ((i * 2654435761u) ^ (i >> 15)) * 2246822519u; }
+
+__device__ __forceinline__ Fp kat_rnd(uint32_t s) {
+  Fp r;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    uint32_t k0 = 2 * j, k1 = 2 * j + 1;
+    uint32_t o0 = k0 == 0 ? 0u : 40503u * k0 + 1u;
+    uint32_t o1 = 40503u * k1 + 1u;
+    uint32_t lo = kat_h(s + o0) & 0xFFFFu;
+    uint32_t hi = kat_h(s + o1) & (k1 == 15 ? 12287u : 0xFFFFu);
+    r.v[j] = lo | (hi << 16);
+  }
+  return r;
+}
+
+__global__ void k_kat_field_mul(uint64_t total, uint32_t* out) {
+  uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint32_t x = 0;
+  for (uint64_t i = tid; i < total; i += stride) {
+    Fp a = kat_rnd((uint32_t)(2 * i));
+    Fp b = kat_rnd((uint32_t)(2 * i + 1));
+    Fp c = Fr::mul(a, b);
+    x ^= c.v[0] & 0xFFFFu;
+  }
+  // wave-level xor reduction, then one atomic per wave
+  for (int off = 32; off > 0; off >>= 1) x ^= __shfl_xor(x, off);
+  if ((threadIdx.x & 63) == 0) atomicXor(out, x);
+}
+
+// ---------------------------------------------------------------- g1_add chain KAT
+__global__ void k_kat_g1_chain(uint64_t k, uint32_t* out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  using J = JacRef<Fq>;
+  Aff<Fq> g = {Fq::one(), Fq::add(Fq::one(), Fq::one())};
+  J::J acc = {g.x, g.y, Fq::one()};
+  for (uint64_t i = 0; i < k; i++) acc = J::madd(acc, g);
+  out[0] = acc.x.v[0] & 0xFFFFu;
+}
+
+// ---------------------------------------------------------------- element-wise ops
+template <class F>
+__device__ typename F::E field_apply(int op, const typename F::E& a, const typename F::E& b);
+
+template <class P>
+__device__ Fp fp_apply(int op, const Fp& a, const Fp& b) {
+  using F = Field<P>;
+  switch (op) {
+    case 0: return F::add(a, b);
+    case 1: return F::sub(a, b);
+    case 2: return F::mul(a, b);
+    case 3: return F::neg(a);
+    case 4: return F::inv(a);
+    case 5: return F::sqr(a);
+    case 6: return F::to_mont(a);
+    default: return F::from_mont(a);
+  }
+}
+
+__device__ Fp2 fp2_apply(int op, const Fp2& a, const Fp2& b) {
+  switch (op) {
+    case 0: return Fq2::add(a, b);
+    case 1: return Fq2::sub(a, b);
+    case 2: return Fq2::mul(a, b);
+    case 3: return Fq2::neg(a);
+    case 4: return Fq2::inv(a);
+    default: return Fq2::sqr(a);
+  }
+}
+
+__global__ void k_field_op(int field, int op, const uint32_t* a, const uint32_t* b, uint32_t* out, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (field == 2) {
+    Fp2 x = load_fp2(a + 16 * i), y = load_fp2(b + 16 * i);
+    store_fp2(out + 16 * i, fp2_apply(op, x, y));
+  } else {
+    Fp x = load_fp(a + 8 * i), y = load_fp(b + 8 * i);
+    Fp r = field == 0 ? fp_apply<FrP>(op, x, y) : fp_apply<FqP>(op, x, y);
+    store_fp(out + 8 * i, r);
+  }
+}
+
+template <class F>
+__global__ void k_curve_op(int op, const uint32_t* p, const uint32_t* q, uint32_t* out, uint64_t n) {
+  using C = Curve<F>;
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int aw = C::aff_words;
+  auto a = C::load_aff(p + aw * i);
+  auto b = C::load_aff(q + aw * i);
+  typename C::P r;
+  switch (op) {
+    case 0: r = C::madd(C::from_affine(a), b); break;
+    case 1: r = C::dbl(C::from_affine(a)); break;
+    case 2: r = C::from_affine(C::aff_neg(a)); break;
+    default: r = C::add(C::from_affine(a), C::madd(C::infinity(), b)); break;
+  }
+  C::store_aff(out + aw * i, C::to_affine(r));
+}
+
+// ---------------------------------------------------------------- throughput probes
+__global__ void k_probe_mad(uint64_t iters, uint32_t seed, uint64_t* sink) {
+  uint32_t t = threadIdx.x + blockIdx.x * blockDim.x;
+  uint64_t acc[8];
+  uint32_t x[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    acc[k] = t + k;
+    x[k] = seed * (k + 1) + t;
+  }
+  uint32_t y = seed ^ t;
+  for (uint64_t it = 0; it < iters; it++) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) acc[k] = (uint64_t)x[k] * y + acc[k];
+#pragma unroll
+    for (int k = 0; k < 8; k++) x[k] ^= (uint32_t)acc[k];
+  }
+  uint64_t s = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) s += acc[k];
+  if (s == 0x1234567ull) sink[0] = s;
+}
+
+__global__ void k_probe_fqmul(uint64_t iters, uint32_t seed, uint32_t* sink) {
+  uint32_t t = threadIdx.x + blockIdx.x * blockDim.x;
+  Fp a[4], b;
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+#pragma unroll
+    for (int j = 0; j < 8; j++) a[k].v[j] = (seed * (j + 7 * k + 1) + t) & (j == 7 ? 0x0fffffffu : 0xffffffffu);
+#pragma unroll
+  for (int j = 0; j < 8; j++) b.v[j] = (seed ^ (j * 977 + t)) & (j == 7 ? 0x0fffffffu : 0xffffffffu);
+  for (uint64_t it = 0; it < iters; it++) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) a[k] = Fq::mul(a[k], b);
+  }
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) s ^= a[k].v[0];
+  if (s == 0x1234567u) sink[0] = s;
+}
+
+__global__ void k_probe_madd(uint64_t iters, uint32_t seed, uint32_t* sink) {
+  uint32_t t = threadIdx.x + blockIdx.x * blockDim.x;
+  // pseudo-random (not on-curve) coordinates: the formula cost is what is measured
+  Xyzz<Fq> acc;
+  Aff<Fq> q;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    uint32_t m = j == 7 ? 0x0fffffffu : 0xffffffffu;
+    acc.X.v[j] = (seed * (j + 1) + t) & m;
+    acc.Y.v[j] = (seed * (j + 3) ^ t) & m;
+    acc.ZZ.v[j] = (seed * (j + 5) + 3 * t) & m;
+    acc.ZZZ.v[j] = (seed * (j + 9) ^ 5 * t) & m;
+    q.x.v[j] = (seed * (j + 11) + 7 * t) & m;
+    q.y.v[j] = (seed * (j + 13) ^ 9 * t) & m;
+  }
+  for (uint64_t it = 0; it < iters; it++) acc = G1::madd(acc, q);
+  if ((acc.X.v[0] ^ acc.ZZZ.v[1]) == 0x1234567u) sink[0] = 1;
+}
+
+}  // namespace bg
+
+using namespace bg;
+
+static void upload(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  BG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+}
+
+extern "C" int bg_kat_field_mul_xor(int log2_count, uint32_t* xor_out) {
+  return guard([&] {
+    BG_REQUIRE(log2_count >= 0 && log2_count <= 31 && xor_out, BG_ERR_ARG, "bg_kat_field_mul_xor: bad argument");
+    Ctx& c = ctx();
+    uint32_t* d = c.scratch[0].as<uint32_t>(1);
+    BG_HIP(hipMemsetAsync(d, 0, 4, c.stream));
+    uint64_t total = 1ull << log2_count;
+    int blocks = (int)std::min<uint64_t>((total + 255) / 256, (uint64_t)c.cu_count * 16);
+    k_kat_field_mul<<<blocks, 256, 0, c.stream>>>(total, d);
+    BG_HIP(hipGetLastError());
+    BG_HIP(hipMemcpyAsync(xor_out, d, 4, hipMemcpyDeviceToHost, c.stream));
+    BG_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+extern "C" int bg_kat_g1_add_chain(uint64_t k, uint32_t* x16_out) {
+  return guard([&] {
+    BG_REQUIRE(x16_out && k <= (1ull << 20), BG_ERR_ARG, "bg_kat_g1_add_chain: bad argument");
+    Ctx& c = ctx();
+    uint32_t* d = c.scratch[0].as<uint32_t>(1);
+    k_kat_g1_chain<<<1, 64, 0, c.stream>>>(k, d);
+    BG_HIP(hipGetLastError());
+    BG_HIP(hipMemcpyAsync(x16_out, d, 4, hipMemcpyDeviceToHost, c.stream));
+    BG_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+extern "C" int bg_field_op(int field, int op, const uint32_t* a, const uint32_t* b, uint32_t* out, uint64_t n) {
+  return guard([&] {
+    BG_REQUIRE(field >= 0 && field <= 2 && op >= 0 && op <= 7 && a && out, BG_ERR_ARG, "bg_field_op: bad argument");
+    BG_REQUIRE(!(field == 2 && op > 5), BG_ERR_ARG, "bg_field_op: Fq2 has no Montgomery conversion op");
+    if (n == 0) return;
+    Ctx& c = ctx();
+    size_t w = field == 2 ? 16 : 8;
+    size_t bytes = n * w * 4;
+    uint32_t* da = c.scratch[0].as<uint32_t>(n * w);
+    uint32_t* db = c.scratch[1].as<uint32_t>(n * w);
+    uint32_t* dout = c.scratch[2].as<uint32_t>(n * w);
+    upload(da, a, bytes, c.stream);
+    upload(db, b ? b : a, bytes, c.stream);
+    k_field_op<<<(unsigned)((n + 127) / 128), 128, 0, c.stream>>>(field, op, da, db, dout, n);
+    BG_HIP(hipGetLastError());
+    BG_HIP(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, c.stream));
+    BG_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+extern "C" int bg_curve_op(int group, int op, const uint32_t* p, const uint32_t* q, uint32_t* out, uint64_t n) {
+  return guard([&] {
+    BG_REQUIRE((group == 1 || group == 2) && op >= 0 && op <= 3 && p && out, BG_ERR_ARG, "bg_curve_op: bad argument");
+    if (n == 0) return;
+    Ctx& c = ctx();
+    size_t w = group == 1 ? 16 : 32;
+    size_t bytes = n * w * 4;
+    uint32_t* dp = c.scratch[0].as<uint32_t>(n * w);
+    uint32_t* dq = c.scratch[1].as<uint32_t>(n * w);
+    uint32_t* dout = c.scratch[2].as<uint32_t>(n * w);
+    upload(dp, p, bytes, c.stream);
+    upload(dq, q ? q : p, bytes, c.stream);
+    unsigned blocks = (unsigned)((n + 63) / 64);
+    if (group == 1)
+      k_curve_op<Fq><<<blocks, 64, 0, c.stream>>>(op, dp, dq, dout, n);
+    else
+      k_curve_op<Fq2><<<blocks, 64, 0, c.stream>>>(op, dp, dq, dout, n);
+    BG_HIP(hipGetLastError());
+    BG_HIP(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, c.stream));
+    BG_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+extern "C" int bg_probe_rate(int kind, uint64_t iters, double* result) {
+  return guard([&] {
+    BG_REQUIRE(kind >= 0 && kind <= 2 && result && iters > 0, BG_ERR_ARG, "bg_probe_rate: bad argument");
+    Ctx& c = ctx();
+    uint32_t* sink = c.scratch[0].as<uint32_t>(4);
+    int threads = 256;
+    int blocks = c.cu_count * 16;
+    hipEvent_t e0, e1;
+    BG_HIP(hipEventCreate(&e0));
+    BG_HIP(hipEventCreate(&e1));
+    auto launch = [&](uint64_t it) {
+      if (kind == 0)
+        k_probe_mad<<<blocks, threads, 0, c.stream>>>(it, 12345u, reinterpret_cast<uint64_t*>(sink));
+      else if (kind == 1)
+        k_probe_fqmul<<<blocks, threads, 0, c.stream>>>(it, 12345u, sink);
+      else
+        k_probe_madd<<<blocks, threads, 0, c.stream>>>(it, 12345u, sink);
+    };
+    launch(1);  // warm-up
+    BG_HIP(hipEventRecord(e0, c.stream));
+    launch(iters);
+    BG_HIP(hipEventRecord(e1, c.stream));
+    BG_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    BG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    double per_thread = kind == 0 ? 8.0 : (kind == 1 ? 4.0 : 1.0);
+    result[0] = (double)blocks * threads * per_thread * (double)iters / (ms * 1e-3);
+    result[1] = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  });
+}

@@ -1,0 +1,132 @@
+// Library context, error reporting and kernel timing (C ABI: bg_init, bg_last_error,
+// bg_set_timing, bg_timing_report, bg_timing_reset).
+#include "bg_runtime.hpp"
+#include <stdarg.h>
+#include <string.h>
+#include <map>
+#include <string>
+
+namespace bg {
+
+static thread_local char g_err[512] = "";
+static std::mutex g_init_mu;
+static Ctx* g_ctx = nullptr;
+static int g_device = 0;
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+}
+
+void ensure_init() {
+  std::lock_guard<std::mutex> lk(g_init_mu);
+  if (g_ctx) return;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) {
+    set_error("no HIP device available (%s)", hipGetErrorString(e));
+    throw Error{BG_ERR_DEVICE};
+  }
+  BG_REQUIRE(g_device >= 0 && g_device < n, BG_ERR_ARG, "device %d out of range (%d devices)", g_device, n);
+  BG_HIP(hipSetDevice(g_device));
+  Ctx* c = new Ctx();
+  c->device = g_device;
+  BG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (auto& s : c->aux) BG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  hipDeviceProp_t prop;
+  BG_HIP(hipGetDeviceProperties(&prop, g_device));
+  c->cu_count = prop.multiProcessorCount;
+  g_ctx = c;
+}
+
+Ctx& ctx() {
+  if (!g_ctx) ensure_init();
+  BG_HIP(hipSetDevice(g_ctx->device));
+  return *g_ctx;
+}
+
+// ---------------------------------------------------------------- timing
+struct TimeAcc {
+  double total_ms = 0;
+  uint64_t count = 0;
+};
+static std::mutex g_tmu;
+static std::map<std::string, TimeAcc> g_times;
+struct Pending {
+  std::string name;
+  hipEvent_t a, b;
+};
+static std::vector<Pending> g_pending;
+
+void timing_record(const char* name, hipStream_t, hipEvent_t start, hipEvent_t stop) {
+  std::lock_guard<std::mutex> lk(g_tmu);
+  g_pending.push_back({name, start, stop});
+}
+
+static void timing_flush() {
+  std::lock_guard<std::mutex> lk(g_tmu);
+  for (auto& p : g_pending) {
+    float ms = 0;
+    if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      auto& t = g_times[p.name];
+      t.total_ms += ms;
+      t.count += 1;
+    }
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  g_pending.clear();
+}
+
+}  // namespace bg
+
+using namespace bg;
+
+extern "C" int bg_init(int device) {
+  return guard([&] {
+    {
+      std::lock_guard<std::mutex> lk(g_init_mu);
+      if (g_ctx && g_ctx->device != device) {
+        set_error("bg_init: already initialised on device %d", g_ctx->device);
+        throw Error{BG_ERR_ARG};
+      }
+      g_device = device;
+    }
+    ensure_init();
+  });
+}
+
+extern "C" const char* bg_last_error(void) { return g_err; }
+
+extern "C" int bg_set_timing(int on) {
+  return guard([&] { ctx().timing = on != 0; });
+}
+
+extern "C" int bg_timing_reset(void) {
+  return guard([&] {
+    timing_flush();
+    std::lock_guard<std::mutex> lk(g_tmu);
+    g_times.clear();
+  });
+}
+
+extern "C" int bg_timing_report(char* names, double* avg_ms, uint64_t* counts, int cap) {
+  int n = 0;
+  int rc = guard([&] {
+    timing_flush();
+    std::lock_guard<std::mutex> lk(g_tmu);
+    for (auto& kv : g_times) {
+      if (n >= cap) break;
+      if (names) {
+        strncpy(names + 48 * n, kv.first.c_str(), 47);
+        names[48 * n + 47] = 0;
+      }
+      if (avg_ms) avg_ms[n] = kv.second.total_ms / (double)kv.second.count;
+      if (counts) counts[n] = kv.second.count;
+      n++;
+    }
+  });
+  return rc == BG_OK ? n : rc;
+}

@@ -1,0 +1,145 @@
+/*
+ * bg16 -- MI355X (gfx950) Groth16/BN254 proving backend: C ABI.
+ *
+ * This is the drop-in boundary for the reference prover's device-dispatched hot path
+ * (bend2/prove.bend:24-49, the three `!` sites at :27, :37, :47): the Pippenger MSMs
+ * of bend2/src/msm.bend and the QAP NTTs of bend2/src/ntt.bend, plus the resident-key
+ * prover around them.  All entry points return an int status (0 = BG_OK, negative =
+ * error class; bg_last_error() gives the message).  No exception or abort crosses the
+ * ABI.  Buffers are caller-owned; inputs are read-only.
+ *
+ * Element encodings (flags):
+ *   BG_FMT_WORDS16  the reference's Array<U32> layout (bend2/FORMAT.md:5-13): one field
+ *                   element = 16 u32 words each holding a 16-bit limb, Montgomery R = 2^256
+ *                   (MSM scalars: canonical).  G1 affine 32 words, G2 affine 64 words,
+ *                   infinity = all zero.
+ *   BG_FMT_PACKED   the same integers packed as 8 x u32 little-endian words (32 B per
+ *                   element; w32[j] = w16[2j] | w16[2j+1] << 16).
+ * Memory (flags):
+ *   BG_MEM_HOST     pointers are host memory (copied in and out by the call)
+ *   BG_MEM_DEVICE   pointers are device memory on the library's device (packed only)
+ */
+#ifndef BG16_H
+#define BG16_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  BG_OK = 0,
+  BG_ERR_ARG = -1,      /* bad argument (size, flag, null pointer) */
+  BG_ERR_DEVICE = -2,   /* HIP runtime error */
+  BG_ERR_NOMEM = -3,    /* device or host allocation failed / over budget */
+  BG_ERR_IO = -4,       /* file missing, truncated or malformed */
+  BG_ERR_INTERNAL = -5
+};
+
+enum {
+  BG_FMT_WORDS16 = 0,
+  BG_FMT_PACKED = 1,
+  BG_MEM_HOST = 0,
+  BG_MEM_DEVICE = 2
+};
+
+/* NTT directions: bend2/src/ntt.bend:431-442 (fft, ifft, coset_fft, coset_ifft) */
+enum { BG_NTT_FFT = 0, BG_NTT_IFFT = 1, BG_NTT_COSET_FFT = 2, BG_NTT_COSET_IFFT = 3 };
+
+/* ------------------------------------------------------------------ library */
+
+/* Selects the HIP device (default 0) and creates the context; idempotent. */
+int bg_init(int device);
+/* Message of the last failing call on this process. */
+const char* bg_last_error(void);
+/* Enables per-kernel HIP-event timing of the hot-path kernels (0/1). */
+int bg_set_timing(int on);
+/* Fills names/avg_ms/counts for up to cap kernels timed since the last reset; returns
+ * the number of entries.  names receives NUL-separated strings (cap * 48 bytes). */
+int bg_timing_report(char* names, double* avg_ms, uint64_t* counts, int cap);
+int bg_timing_reset(void);
+
+/* ------------------------------------------------------------------ MSM
+ * Replaces Msm.g1(points, scalars, n) / Msm.g2 (bend2/src/msm.bend:297, :529) and
+ * Msm.g1_with / g2_with (:293, :525): out = sum_{i<n} scalars[i] * points[i].
+ * points: n affine Montgomery points; scalars: n canonical Fr (any value < 2^256 is
+ * read as an integer and reduced mod r, which leaves the result unchanged for points of
+ * order r).  out: one affine point (same encoding as points; infinity = zeros).  The
+ * reference returns Jacobian; the affine point is unique, and the reference's caller
+ * normalises it (bend2/src/groth16.bend:281-293).
+ * window_bits: 0 = tuned default, else the signed-digit window width c (4..20). */
+int bg_msm_g1(const uint32_t* points, const uint32_t* scalars, uint64_t n, uint32_t* out,
+              int flags, int window_bits);
+int bg_msm_g2(const uint32_t* points, const uint32_t* scalars, uint64_t n, uint32_t* out,
+              int flags, int window_bits);
+
+/* ------------------------------------------------------------------ NTT
+ * Replaces Ntt.fft / ifft / coset_fft / coset_ifft (bend2/src/ntt.bend:431-442), in
+ * place on 2^log2n Montgomery Fr elements, natural order in and out.  ifft scales by
+ * n^-1; the coset generator is 5. */
+int bg_ntt(uint32_t* data, int log2n, int direction, int flags);
+
+/* Replaces Ntt.qap_h(a, b, c, log2) (bend2/src/ntt.bend:459):
+ * h = coset_ifft((coset_fft(ifft a) * coset_fft(ifft b) - coset_fft(ifft c)) / (g^d - 1)),
+ * d = 2^log2n Montgomery coefficients, natural order. */
+int bg_qap_h(const uint32_t* a, const uint32_t* b, const uint32_t* c, int log2n, uint32_t* h,
+             int flags);
+
+/* ------------------------------------------------------------------ resident-key prover
+ * Reproduces bend2/prove.bend's Pr.run (:129-162) on the device: evaluation vectors
+ * (groth16.bend:162-208), qap_h, the five MSMs in the reference's order and slots
+ * (prove.bend:64-116) and the final combination (groth16.bend:298-308). */
+typedef struct bg_pk bg_pk;
+
+/* Loads a FORMAT.md key directory (header.bin, r1cs_{a,b,c}.bin, pk_*.bin) into HBM. */
+int bg_pk_load(const char* dir, bg_pk** out);
+int bg_pk_free(bg_pk* pk);
+/* header: num_constraints, num_instance, num_witness, domain_log2, n_a, n_b, n_h, n_l */
+int bg_pk_header(const bg_pk* pk, uint32_t* header8);
+/* Device bytes held by a loaded key. */
+int bg_pk_device_bytes(const bg_pk* pk, uint64_t* bytes);
+
+/* One proof.  witness: num_instance + num_witness Fr (Montgomery, FORMAT.md witness.bin
+ * order); rs: r then s (Montgomery, FORMAT.md rs.bin).  proof_out: A (G1), B (G2),
+ * C (G1) affine Montgomery (the proof_ref.bin layout), in the flags' encoding (host).
+ * t_ms (optional, 8 doubles): upload, evals, qap, msm_a, msm_b1, msm_b2, msm_h+msm_l,
+ * finish -- phase wall times. */
+int bg_prove(bg_pk* pk, const uint32_t* witness, const uint32_t* rs, uint32_t* proof_out,
+             int flags, double* t_ms);
+
+/* Proof as the reference prints it (bend2/FORMAT.md:33): "A x y\nB x0 x1 y0 y1\nC x y\n"
+ * with canonical big-endian 0x hex; proof in BG_FMT_PACKED.  Returns bytes written. */
+int bg_proof_format(const uint32_t* proof_packed, char* buf, int cap);
+
+/* ------------------------------------------------------------------ key generation
+ * Trapdoor Groth16 setup for SquareChain(2^log2) (ref/src/circuit.rs:37-73,
+ * ref/src/gen.rs:65-164 restated with this library's own seeded generator): writes the
+ * FORMAT.md files under dir plus trapdoor.bin (tau, alpha, beta, gamma, delta as
+ * Montgomery Fr) for closed-form checks.  threads <= 0 uses all host cores. */
+int bg_setup_square_chain(const char* dir, int log2, uint64_t seed, int threads);
+
+/* ------------------------------------------------------------------ diagnostics
+ * Known-answer and element-wise kernels used by the parity tests (host buffers, packed). */
+
+/* bend2/bench/field_mul.bend:9-31: xor of limb 0 of 2^N hash-generated Fr products. */
+int bg_kat_field_mul_xor(int log2_count, uint32_t* xor_out);
+/* bend2/bench/g1_add.bend: k reference-formula mixed adds acc <- acc + G from G; returns
+ * limb 0 (16 bits) of the Jacobian X in Montgomery form. */
+int bg_kat_g1_add_chain(uint64_t k, uint32_t* x16_out);
+
+/* field: 0 Fr, 1 Fq, 2 Fq2.  op: 0 add, 1 sub, 2 mul, 3 neg, 4 inv, 5 sqr, 6 to_mont,
+ * 7 from_mont.  n elements of a (and b for binary ops) -> out. */
+int bg_field_op(int field, int op, const uint32_t* a, const uint32_t* b, uint32_t* out,
+                uint64_t n);
+/* group: 1 G1, 2 G2.  op: 0 add (p + q), 1 double (2p), 2 neg (-p), 3 add through XYZZ
+ * full add (from_affine(p) + from_affine(q)).  Affine in, affine out. */
+int bg_curve_op(int group, int op, const uint32_t* p, const uint32_t* q, uint32_t* out,
+                uint64_t n);
+/* Throughput probes: kind 0 = v_mad_u64_u32 issue rate (ops/s), 1 = Fq Montgomery
+ * products/s, 2 = G1 XYZZ mixed adds/s.  result[0] = rate, result[1] = checksum. */
+int bg_probe_rate(int kind, uint64_t iters, double* result);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,432 @@
+"""CPU ORACLE -- test infrastructure only, never part of the product path.
+
+Python side of the oracle: a ctypes binding of ``liboracle.so`` (the C restatement in
+``bn254_oracle.c``) plus a small pure-Python big-integer restatement used to pin it.
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may
+import this module, and only as the checker (or the timed CPU baseline).
+
+Pins (what each oracle function is checked against; the tests named here hold them):
+
+=====================  =====================================================  ===================
+oracle function        pin                                                    test
+=====================  =====================================================  ===================
+f_mul (64-bit CIOS)    reference field_mul xor KATs N=18,20,22 (36312, 21587,  test_oracle_kat
+                       18660; bend2/probe/gpu_rounds.out.txt:3,19,
+                       bend2/bench/box_results.md:29); equality with the
+                       16-bit restatement of gen_field.py; Python pow
+fe16_mul               same KATs; committed constants one/r2/two/neg_one of   test_oracle_kat
+                       bend2/src/fr.bend:14-20 and bend2/proofs/
+                       fr_concrete.bend:9-16
+f_inv, to/from mont    Python pow(x, -1, p); x*2^256 mod p                   test_oracle_field
+NTT constants          bend2/src/ntt.bend:22-41 (root, root_inv, g, g^-1,     test_oracle_kat
+                       2^-1 as Montgomery 16-bit limbs)
+Fq2 mul/sqr/inv        Python closed form with u^2 = -1                      test_oracle_field
+G1 Jacobian formulas   g1_add KATs (sum=41096 at N=14 FD=0, 917307392 at     test_oracle_kat
+                       N=20 FD=14, 1523843072 at N=20 FD=16, 2728394752 at
+                       N=20 FD=18; gpu_rounds.out.txt:7, box_results.md:41-51)
+G1/G2 group law        Python affine textbook law; on-curve y^2 = x^3 + b;    test_oracle_curve
+                       special cases inf, P = Q, P = -Q; r*G = inf
+MSM                    brute force sum k_i P_i (Python); closed form          test_oracle_msm
+                       (sum k_i g_i) G with P_i = g_i G
+NTT / qap_h            naive O(n^2) DFT (Python); ifft(fft) = id; QAP         test_oracle_ntt
+                       identity A B - C = h (x^d - 1) at a random x, h[d-1]=0
+prove                  trapdoor closed form of A, B, C (setup secrets known)  test_oracle_prove
+                       and the Groth16 verification equation in the exponent
+=====================  =====================================================  ===================
+
+The arkworks leg (seed-42 pk and proof_ref.bin) cannot be produced in this environment
+(no cargo / crates), so bit-equality with arkworks' own seed-42 proof is parity unpinned;
+the closed forms above fix every output bit mathematically instead.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+P_R = 21888242871839275222246405745257275088548364400416034343698204186575808495617
+P_Q = 21888242871839275222246405745257275088696311157297823662689037894645226208583
+ROOT28 = 19103219067921713944291392827692070036145651957329286315305642004821462161904
+GEN = 5
+R256 = 1 << 256
+MASK64 = (1 << 64) - 1
+
+# BN254 generators.  G1 = (1, 2); G2 is the standard generator of the order-r subgroup
+# of the D-type twist y^2 = x^3 + 3/(9+u).  test_oracle_curve checks both are on their
+# curves and have order r.
+G1_GEN = (1, 2)
+G2_GEN = (
+    (10857046999023057135944570762232829481370756359578518086990519993285655852781,
+     11559732032986387107991004021392285783925812861821192530917403151452391805634),
+    (8495653923123431417604973247489272438418190587263600148770280649306958101930,
+     4082367875863433681332203403145435568316851327593401208105741076214120093531),
+)
+
+
+# ----------------------------------------------------------------- int <-> words
+
+def to_words(x: int) -> list[int]:
+    return [(x >> (64 * i)) & MASK64 for i in range(4)]
+
+
+def from_words(w) -> int:
+    return sum(int(w[i]) << (64 * i) for i in range(4))
+
+
+def mont(x: int, p: int) -> int:
+    return x * R256 % p
+
+
+def unmont(x: int, p: int) -> int:
+    return x * pow(R256, -1, p) % p
+
+
+def ints_to_array(xs, mont_p: int | None = None) -> np.ndarray:
+    """Pack ints into an (n, 4) uint64 array (optionally converting to Montgomery form)."""
+    out = np.zeros((len(xs), 4), dtype=np.uint64)
+    for i, x in enumerate(xs):
+        v = mont(x, mont_p) if mont_p else x
+        for k in range(4):
+            out[i, k] = (v >> (64 * k)) & MASK64
+    return out
+
+
+def array_to_ints(a: np.ndarray, unmont_p: int | None = None) -> list[int]:
+    a = np.asarray(a, dtype=np.uint64).reshape(-1, 4)
+    res = []
+    for row in a:
+        v = from_words(row)
+        res.append(unmont(v, unmont_p) if unmont_p else v)
+    return res
+
+
+def words16(x: int) -> list[int]:
+    return [(x >> (16 * j)) & 0xFFFF for j in range(16)]
+
+
+# ------------------------------------------------------------- pure-Python restatement
+
+class PyField:
+    """Textbook prime field on Python ints (canonical representatives)."""
+
+    def __init__(self, p):
+        self.p = p
+
+    def inv(self, a):
+        return pow(a, -1, self.p) if a % self.p else 0
+
+
+def fq2_mul(a, b, p=P_Q):
+    return ((a[0] * b[0] - a[1] * b[1]) % p, (a[0] * b[1] + a[1] * b[0]) % p)
+
+
+def fq2_add(a, b, p=P_Q):
+    return ((a[0] + b[0]) % p, (a[1] + b[1]) % p)
+
+
+def fq2_sub(a, b, p=P_Q):
+    return ((a[0] - b[0]) % p, (a[1] - b[1]) % p)
+
+
+def fq2_inv(a, p=P_Q):
+    n = (a[0] * a[0] + a[1] * a[1]) % p
+    ni = pow(n, -1, p) if n else 0
+    return (a[0] * ni % p, (-a[1]) * ni % p)
+
+
+B_G1 = 3
+B_G2 = fq2_mul((3, 0), fq2_inv((9, 1)))
+
+
+def g1_on_curve(pt):
+    if pt is None:
+        return True
+    x, y = pt
+    return (y * y - x * x * x - B_G1) % P_Q == 0
+
+
+def g2_on_curve(pt):
+    if pt is None:
+        return True
+    x, y = pt
+    lhs = fq2_mul(y, y)
+    rhs = fq2_add(fq2_mul(fq2_mul(x, x), x), B_G2)
+    return lhs == rhs
+
+
+def g1_add_aff(a, b):
+    """Affine chord-and-tangent law over Fq; None = infinity."""
+    p = P_Q
+    if a is None:
+        return b
+    if b is None:
+        return a
+    (x1, y1), (x2, y2) = a, b
+    if x1 == x2:
+        if (y1 + y2) % p == 0:
+            return None
+        lam = 3 * x1 * x1 * pow(2 * y1, -1, p) % p
+    else:
+        lam = (y2 - y1) * pow(x2 - x1, -1, p) % p
+    x3 = (lam * lam - x1 - x2) % p
+    return (x3, (lam * (x1 - x3) - y1) % p)
+
+
+def g2_add_aff(a, b):
+    if a is None:
+        return b
+    if b is None:
+        return a
+    (x1, y1), (x2, y2) = a, b
+    if x1 == x2:
+        if fq2_add(y1, y2) == (0, 0):
+            return None
+        lam = fq2_mul(fq2_mul((3, 0), fq2_mul(x1, x1)), fq2_inv(fq2_add(y1, y1)))
+    else:
+        lam = fq2_mul(fq2_sub(y2, y1), fq2_inv(fq2_sub(x2, x1)))
+    x3 = fq2_sub(fq2_sub(fq2_mul(lam, lam), x1), x2)
+    return (x3, fq2_sub(fq2_mul(lam, fq2_sub(x1, x3)), y1))
+
+
+def _mul_aff(add, pt, k):
+    acc = None
+    for bit in bin(k)[2:] if k > 0 else "":
+        acc = add(acc, acc)
+        if bit == "1":
+            acc = add(acc, pt)
+    return acc
+
+
+def g1_mul_aff(pt, k):
+    return _mul_aff(g1_add_aff, pt, k)
+
+
+def g2_mul_aff(pt, k):
+    return _mul_aff(g2_add_aff, pt, k)
+
+
+def naive_dft(a, w, p=P_R):
+    n = len(a)
+    return [sum(a[j] * pow(w, j * k, p) for j in range(n)) % p for k in range(n)]
+
+
+def omega(log2n, inverse=False):
+    w = pow(ROOT28, 1 << (28 - log2n), P_R)
+    return pow(w, -1, P_R) if inverse else w
+
+
+# --------------------------------------------------------------- point packing
+
+def g1_pack(pts) -> np.ndarray:
+    """affine G1 (ints or None) -> (n, 8) uint64 Montgomery, infinity = zeros."""
+    out = np.zeros((len(pts), 8), dtype=np.uint64)
+    for i, pt in enumerate(pts):
+        if pt is None:
+            continue
+        for k, v in enumerate(pt):
+            m = mont(v, P_Q)
+            for w in range(4):
+                out[i, 4 * k + w] = (m >> (64 * w)) & MASK64
+    return out
+
+
+def g1_unpack(a) -> list:
+    a = np.asarray(a, dtype=np.uint64).reshape(-1, 8)
+    res = []
+    for row in a:
+        x, y = from_words(row[:4]), from_words(row[4:])
+        res.append(None if x == 0 and y == 0 else (unmont(x, P_Q), unmont(y, P_Q)))
+    return res
+
+
+def g2_pack(pts) -> np.ndarray:
+    out = np.zeros((len(pts), 16), dtype=np.uint64)
+    for i, pt in enumerate(pts):
+        if pt is None:
+            continue
+        vals = [pt[0][0], pt[0][1], pt[1][0], pt[1][1]]
+        for k, v in enumerate(vals):
+            m = mont(v, P_Q)
+            for w in range(4):
+                out[i, 4 * k + w] = (m >> (64 * w)) & MASK64
+    return out
+
+
+def g2_unpack(a) -> list:
+    a = np.asarray(a, dtype=np.uint64).reshape(-1, 16)
+    res = []
+    for row in a:
+        v = [from_words(row[4 * k:4 * k + 4]) for k in range(4)]
+        if all(x == 0 for x in v):
+            res.append(None)
+        else:
+            v = [unmont(x, P_Q) for x in v]
+            res.append(((v[0], v[1]), (v[2], v[3])))
+    return res
+
+
+# ------------------------------------------------------------------ ctypes binding
+
+_lib = None
+
+
+def lib():
+    """Load liboracle.so (built by oracle/Makefile via __graft_entry__.build())."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built; run `make -C oracle` or __graft_entry__.build()")
+        L = ctypes.CDLL(str(LIB_PATH))
+        P = ctypes.c_void_p
+        U64 = ctypes.c_uint64
+        sig = {
+            "or_f_add": [ctypes.c_int, P, P, P], "or_f_sub": [ctypes.c_int, P, P, P],
+            "or_f_neg": [ctypes.c_int, P, P], "or_f_mul": [ctypes.c_int, P, P, P],
+            "or_f_inv": [ctypes.c_int, P, P], "or_f_to_mont": [ctypes.c_int, P, P],
+            "or_f_from_mont": [ctypes.c_int, P, P],
+            "or_f_mul_vec": [ctypes.c_int, P, P, P, U64],
+            "or_f_from_mont_vec": [ctypes.c_int, P, P, U64],
+            "or_f_to_mont_vec": [ctypes.c_int, P, P, U64],
+            "or_fe16_mul": [ctypes.c_int, P, P, P],
+            "or_f_consts": [ctypes.c_int, P, P, P, P],
+            "or_field_mul_xor": [ctypes.c_int, ctypes.c_int],
+            "or_kat_rnd": [ctypes.c_uint32, P],
+            "or_g1_add_chain_x16": [U64], "or_g1_add_chain": [U64, P],
+            "or_fq2_mul": [P, P, P], "or_fq2_sqr": [P, P], "or_fq2_inv": [P, P],
+            "or_g1_add": [P, P, P], "or_g1_add_mixed": [P, P, P], "or_g1_double": [P, P],
+            "or_g1_to_affine": [P, P], "or_g1_from_affine": [P, P],
+            "or_g2_add": [P, P, P], "or_g2_add_mixed": [P, P, P], "or_g2_double": [P, P],
+            "or_g2_to_affine": [P, P], "or_g2_from_affine": [P, P],
+            "or_g1_mul_aff": [P, P, P], "or_g2_mul_aff": [P, P, P],
+            "or_msm_g1": [P, P, P, U64, ctypes.c_int], "or_msm_g2": [P, P, P, U64, ctypes.c_int],
+            "or_ntt": [P, ctypes.c_int, ctypes.c_int], "or_omega": [P, ctypes.c_int, ctypes.c_int],
+            "or_root28": [P],
+            "or_qap_h": [P, P, P, P, ctypes.c_int],
+            "or_prove": [P] * 23 + [P],
+            "or_threads": [], "or_set_threads": [ctypes.c_int],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = None
+        L.or_field_mul_xor.restype = ctypes.c_uint32
+        L.or_g1_add_chain_x16.restype = ctypes.c_uint32
+        L.or_prove.restype = ctypes.c_int
+        L.or_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def f_binop(name: str, fid: int, a: int, b: int) -> int:
+    x = np.array(to_words(a), dtype=np.uint64)
+    y = np.array(to_words(b), dtype=np.uint64)
+    r = np.zeros(4, dtype=np.uint64)
+    getattr(lib(), name)(fid, _ptr(r), _ptr(x), _ptr(y))
+    return from_words(r)
+
+
+def f_unop(name: str, fid: int, a: int) -> int:
+    x = np.array(to_words(a), dtype=np.uint64)
+    r = np.zeros(4, dtype=np.uint64)
+    getattr(lib(), name)(fid, _ptr(r), _ptr(x))
+    return from_words(r)
+
+
+def fe16_mul(fid: int, a16, b16):
+    a = np.array(a16, dtype=np.uint32)
+    b = np.array(b16, dtype=np.uint32)
+    o = np.zeros(16, dtype=np.uint32)
+    lib().or_fe16_mul(fid, _ptr(o), _ptr(a), _ptr(b))
+    return [int(v) for v in o]
+
+
+def field_mul_xor(n: int, use16: bool = False) -> int:
+    return int(lib().or_field_mul_xor(n, 1 if use16 else 0))
+
+
+def kat_rnd(s: int):
+    o = np.zeros(16, dtype=np.uint32)
+    lib().or_kat_rnd(s, _ptr(o))
+    return [int(v) for v in o]
+
+
+def g1_add_chain_x16(k: int) -> int:
+    return int(lib().or_g1_add_chain_x16(k))
+
+
+def msm_g1(points: np.ndarray, scalars: np.ndarray, c: int = 0) -> np.ndarray:
+    """points (n, 8) u64 affine Montgomery, scalars (n, 4) canonical -> (8,) affine."""
+    points = np.ascontiguousarray(points, dtype=np.uint64)
+    scalars = np.ascontiguousarray(scalars, dtype=np.uint64)
+    n = scalars.shape[0]
+    out = np.zeros(8, dtype=np.uint64)
+    lib().or_msm_g1(_ptr(out), _ptr(points), _ptr(scalars), n, c)
+    return out
+
+
+def msm_g2(points: np.ndarray, scalars: np.ndarray, c: int = 0) -> np.ndarray:
+    points = np.ascontiguousarray(points, dtype=np.uint64)
+    scalars = np.ascontiguousarray(scalars, dtype=np.uint64)
+    n = scalars.shape[0]
+    out = np.zeros(16, dtype=np.uint64)
+    lib().or_msm_g2(_ptr(out), _ptr(points), _ptr(scalars), n, c)
+    return out
+
+
+def ntt(a: np.ndarray, log2n: int, direction: int) -> np.ndarray:
+    """direction: 0 fft, 1 ifft, 2 coset_fft, 3 coset_ifft (Montgomery Fr, (n, 4) u64)."""
+    x = np.ascontiguousarray(a, dtype=np.uint64).copy()
+    assert x.shape[0] == 1 << log2n
+    lib().or_ntt(_ptr(x), log2n, direction)
+    return x
+
+
+def qap_h(a: np.ndarray, b: np.ndarray, c: np.ndarray, log2n: int) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.uint64).copy()
+    b = np.ascontiguousarray(b, dtype=np.uint64).copy()
+    c = np.ascontiguousarray(c, dtype=np.uint64).copy()
+    h = np.zeros_like(a)
+    lib().or_qap_h(_ptr(h), _ptr(a), _ptr(b), _ptr(c), log2n)
+    return h
+
+
+def prove(inst: dict):
+    """Full CPU prove of an instance dict (keys as produced by tests' loader).
+
+    Returns (A (8,), B (16,), C (8,)) affine Montgomery u64 arrays and phase times.
+    """
+    L = lib()
+    A = np.zeros(8, dtype=np.uint64)
+    B = np.zeros(16, dtype=np.uint64)
+    C = np.zeros(8, dtype=np.uint64)
+    t = np.zeros(7, dtype=np.float64)
+    hdr = np.ascontiguousarray(inst["header"], dtype=np.uint32)
+    arrs = [hdr, inst["z"],
+            inst["a_off"], inst["a_col"], inst["a_coef"],
+            inst["b_off"], inst["b_col"], inst["b_coef"],
+            inst["c_off"], inst["c_col"], inst["c_coef"],
+            inst["pk_g1"], inst["pk_g2"], inst["pk_a"], inst["pk_b1"], inst["pk_b2"],
+            inst["pk_h"], inst["pk_l"], inst["r"], inst["s"]]
+    arrs = [np.ascontiguousarray(x) for x in arrs]
+    rc = L.or_prove(*[_ptr(x) for x in arrs], _ptr(A), _ptr(B), _ptr(C), _ptr(t))
+    if rc != 0:
+        raise ValueError(f"or_prove rejected the instance (rc={rc})")
+    return A, B, C, t
+
+
+def set_threads(n: int) -> None:
+    lib().or_set_threads(n)
+
+
+def threads() -> int:
+    return int(lib().or_threads())

@@ -1,0 +1,128 @@
+"""GPU parity of the gfx950 field and curve kernels against the reference's known-answer
+values and the CPU oracle (bit-exact)."""
+import json
+import random
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from helpers import g1_points, g2_points, ints_u32, u32_ints, rand_fe
+
+pytestmark = pytest.mark.gpu
+KATS = json.loads((Path(__file__).parent / "golden" / "reference_kats.json").read_text())
+
+
+@pytest.mark.parametrize("n", ["20", "22", "24", "26"])
+def test_field_mul_xor_kat(bg, n):
+    print(f"field_mul KAT N={n}", flush=True)
+    assert bg.kat_field_mul_xor(int(n)) == KATS["field_mul_xor"][n]["xor"]
+
+
+@pytest.mark.parametrize("k,fd,sum_", [(c["N"], c["FD"], c["sum"]) for c in KATS["g1_add_sum"]["cases"]
+                                        if c["N"] - c["FD"] <= 14])
+def test_g1_add_chain_kat(bg, k, fd, sum_):
+    x16 = bg.kat_g1_add_chain(1 << (k - fd))
+    assert (x16 << fd) % (1 << 32) == sum_
+
+
+@pytest.mark.parametrize("fid,p", [(0, O.P_R), (1, O.P_Q)], ids=["Fr", "Fq"])
+def test_field_ops(bg, fid, p):
+    rnd = random.Random(11 + fid)
+    n = 512
+    a = rand_fe(rnd, p, n)
+    b = rand_fe(rnd, p, n)
+    # edge values
+    a[:6] = [0, 1, p - 1, p - 2, 2, (1 << 255) % p]
+    b[:6] = [0, p - 1, p - 1, 1, p - 2, 0]
+    A, B = ints_u32(a), ints_u32(b)
+    R = pow(1 << 256, -1, p)
+    checks = {
+        0: lambda x, y: (x + y) % p,
+        1: lambda x, y: (x - y) % p,
+        2: lambda x, y: x * y * R % p,
+        3: lambda x, y: (-x) % p,
+        4: lambda x, y: pow(x * R, -1, p) * (1 << 256) % p if x else 0,  # Montgomery inverse
+        5: lambda x, y: x * x * R % p,
+        6: lambda x, y: x * (1 << 256) % p,
+        7: lambda x, y: x * R % p,
+    }
+    for op, f in checks.items():
+        got = u32_ints(bg.field_op(fid, op, A, B))
+        want = [f(x, y) for x, y in zip(a, b)]
+        assert got == want, f"op {op}"
+
+
+def test_fq2_ops(bg):
+    rnd = random.Random(5)
+    n = 128
+    p = O.P_Q
+    a = [(rnd.randrange(p), rnd.randrange(p)) for _ in range(n)]
+    b = [(rnd.randrange(p), rnd.randrange(p)) for _ in range(n)]
+    a[0] = (0, 0)
+    enc = lambda v: ints_u32([c for x in v for c in x], p).reshape(-1, 16)
+    dec = lambda arr: [tuple(x) for x in np.array(u32_ints(arr, p), dtype=object).reshape(-1, 2)]
+    A, B = enc(a), enc(b)
+    assert dec(bg.field_op(2, 2, A, B)) == [O.fq2_mul(x, y) for x, y in zip(a, b)]
+    assert dec(bg.field_op(2, 5, A, A)) == [O.fq2_mul(x, x) for x in a]
+    assert dec(bg.field_op(2, 4, A, A)) == [O.fq2_inv(x) for x in a]
+    assert dec(bg.field_op(2, 0, A, B)) == [O.fq2_add(x, y) for x, y in zip(a, b)]
+    assert dec(bg.field_op(2, 1, A, B)) == [O.fq2_sub(x, y) for x, y in zip(a, b)]
+
+
+def _special_pairs(pts, ks, n):
+    """p, q arrays covering q = p, q = -p, p = inf, q = inf and generic pairs."""
+    P = pts.copy()
+    Q = np.roll(pts, 1, axis=0).copy()
+    half = P.shape[1] // 2
+    Q[0] = P[0]                       # P == Q -> doubling path
+    Q[1] = P[1]
+    Q[1, half:] = 0                   # build -P below
+    P[2] = 0                          # P = infinity
+    Q[3] = 0                          # Q = infinity
+    P[4] = 0; Q[4] = 0                # both infinity
+    return P, Q
+
+
+def test_g1_curve_ops(bg):
+    rnd = random.Random(3)
+    n = 48
+    pts, ks = g1_points(rnd, n)
+    P, Q = _special_pairs(pts, ks, n)
+    # Q[1] = -P[1]
+    negp = bg.curve_op(1, 2, P[1:2])
+    Q[1] = negp[0]
+    Pa, Qa = O.g1_unpack(P.view(np.uint64)), O.g1_unpack(Q.view(np.uint64))
+    assert Qa[1] == (Pa[1][0], (-Pa[1][1]) % O.P_Q)
+    for op in (0, 3):
+        got = O.g1_unpack(bg.curve_op(1, op, P, Q).view(np.uint64))
+        want = [O.g1_add_aff(x, y) for x, y in zip(Pa, Qa)]
+        assert got == want, f"op {op}"
+    got = O.g1_unpack(bg.curve_op(1, 1, P).view(np.uint64))
+    assert got == [O.g1_add_aff(x, x) for x in Pa]
+
+
+def test_g2_curve_ops(bg):
+    rnd = random.Random(4)
+    n = 16
+    pts, ks = g2_points(rnd, n)
+    P, Q = _special_pairs(pts, ks, n)
+    Q[1] = bg.curve_op(2, 2, P[1:2])[0]
+    Pa, Qa = O.g2_unpack(P.view(np.uint64)), O.g2_unpack(Q.view(np.uint64))
+    for op in (0, 3):
+        got = O.g2_unpack(bg.curve_op(2, op, P, Q).view(np.uint64))
+        want = [O.g2_add_aff(x, y) for x, y in zip(Pa, Qa)]
+        assert got == want, f"op {op}"
+    got = O.g2_unpack(bg.curve_op(2, 1, P).view(np.uint64))
+    assert got == [O.g2_add_aff(x, x) for x in Pa]
+
+
+def test_probe_rates(bg):
+    """Throughput probes (printed; used for the roofline peak)."""
+    mad, ms0 = bg.probe_rate(0, 4096)
+    fq, ms1 = bg.probe_rate(1, 256)
+    madd, ms2 = bg.probe_rate(2, 64)
+    print(f"\nv_mad_u64_u32: {mad / 1e12:.2f} T/s ({ms0:.2f} ms); Fq mul: {fq / 1e9:.1f} G/s ({ms1:.2f} ms); "
+          f"G1 madd: {madd / 1e9:.2f} G/s ({ms2:.2f} ms)", flush=True)
+    assert mad > 0 and fq > 0 and madd > 0
