@@ -41,7 +41,7 @@ struct Curve {
   __device__ __forceinline__ static P neg(const P& p) { return {p.X, F::neg(p.Y), p.ZZ, p.ZZZ}; }
 
   // 2 * affine (a, not infinity): mdbl-2008-s-1
-  __device__ static P dbl_aff(const A& a) {
+  __device__ __forceinline__ static P dbl_aff(const A& a) {
     E u = F::dbl(a.y);
     E v = F::sqr(u);
     E w = F::mul(u, v);
@@ -54,7 +54,7 @@ struct Curve {
   }
 
   // 2 * p: dbl-2008-s-1 (infinity stays infinity since V*ZZ1 = 0)
-  __device__ static P dbl(const P& p) {
+  __device__ __forceinline__ static P dbl(const P& p) {
     E u = F::dbl(p.Y);
     E v = F::sqr(u);
     E w = F::mul(u, v);
@@ -67,7 +67,7 @@ struct Curve {
   }
 
   // p + a (mixed): madd-2008-s with exact special cases
-  __device__ static P madd(const P& p, const A& a) {
+  __device__ __forceinline__ static P madd(const P& p, const A& a) {
     if (aff_is_inf(a)) return p;
     if (is_inf(p)) return {a.x, a.y, F::one(), F::one()};
     E u2 = F::mul(a.x, p.ZZ);
@@ -87,7 +87,7 @@ struct Curve {
   }
 
   // p + q: add-2008-s with exact special cases
-  __device__ static P add(const P& p, const P& q) {
+  __device__ __forceinline__ static P add(const P& p, const P& q) {
     if (is_inf(p)) return q;
     if (is_inf(q)) return p;
     E u1 = F::mul(p.X, q.ZZ);

@@ -234,6 +234,9 @@ extern "C" int bg_field_op(int field, int op, const uint32_t* a, const uint32_t*
 extern "C" int bg_curve_op(int group, int op, const uint32_t* p, const uint32_t* q, uint32_t* out, uint64_t n) {
   return guard([&] {
     BG_REQUIRE((group == 1 || group == 2) && op >= 0 && op <= 3 && p && out, BG_ERR_ARG, "bg_curve_op: bad argument");
+    // The G2 variant of this diagnostic kernel took an illegal memory access on gfx950
+    // with no cause found in its code; it stays disabled until one is shown.
+    BG_REQUIRE(group == 1, BG_ERR_ARG, "bg_curve_op: the G2 diagnostic kernel is disabled");
     if (n == 0) return;
     Ctx& c = ctx();
     size_t w = group == 1 ? 16 : 32;
@@ -244,10 +247,7 @@ extern "C" int bg_curve_op(int group, int op, const uint32_t* p, const uint32_t*
     upload(dp, p, bytes, c.stream);
     upload(dq, q ? q : p, bytes, c.stream);
     unsigned blocks = (unsigned)((n + 63) / 64);
-    if (group == 1)
-      k_curve_op<Fq><<<blocks, 64, 0, c.stream>>>(op, dp, dq, dout, n);
-    else
-      k_curve_op<Fq2><<<blocks, 64, 0, c.stream>>>(op, dp, dq, dout, n);
+    k_curve_op<Fq><<<blocks, 64, 0, c.stream>>>(op, dp, dq, dout, n);
     BG_HIP(hipGetLastError());
     BG_HIP(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, c.stream));
     BG_HIP(hipStreamSynchronize(c.stream));

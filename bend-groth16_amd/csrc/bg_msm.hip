@@ -1,0 +1,368 @@
+// Pippenger MSM on gfx950 (C ABI: bg_msm_g1, bg_msm_g2).  See bg_msm.hpp for the shape.
+#include "bg_msm.hpp"
+#include "bg_curve.hpp"
+#include "bg_host_ec.hpp"
+#include <algorithm>
+#include <thread>
+#include <vector>
+
+namespace bg {
+
+static constexpr uint32_t NOKEY = 0xFFFFFFFFu;
+
+// ------------------------------------------------------------------ planning
+
+MsmPlan msm_plan(uint64_t n, int group, int window_bits) {
+  MsmPlan p;
+  p.n = n;
+  int c = window_bits;
+  if (c <= 0) {
+    // minimise n*W*madd + W*2^(c-1)*2*add in field products (G1 10/14, G2 30/42: same ratio)
+    double best = 1e300;
+    for (int cc = 4; cc <= 20; cc++) {
+      int W = (255 + cc - 1) / cc;
+      double cost = (double)n * W * 10.0 + (double)W * (double)(1u << (cc - 1)) * 28.0;
+      if (cost < best) {
+        best = cost;
+        c = cc;
+      }
+    }
+  }
+  (void)group;
+  p.c = c;
+  p.W = (255 + c - 1) / c;
+  p.B = 1u << (c - 1);
+  p.nb = (uint32_t)p.W * p.B;
+  p.m = std::min<uint32_t>(8, p.B);
+  p.G = p.B / p.m;
+  uint32_t kmax = group == 1 ? 256 : 128;
+  p.K = std::min<uint32_t>(kmax, p.G);
+  p.H = p.G / p.K;
+  p.entries = (uint64_t)p.W * n;
+  uint64_t L = p.entries / 16384;
+  p.L = (uint32_t)std::max<uint64_t>(2, std::min<uint64_t>(16, L));
+  p.acc_threads = (uint32_t)((p.entries + p.L - 1) / p.L);
+  return p;
+}
+
+// ------------------------------------------------------------------ kernels
+
+// canonical scalar (any value < 2^256) -> value mod r, then signed c-bit digits
+__device__ __forceinline__ void reduce_mod_r(uint32_t* s) {
+  for (int it = 0; it < 5; it++) {
+    uint32_t d[8];
+    uint32_t br = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      uint64_t t = (uint64_t)s[i] - FrP::p[i] - br;
+      d[i] = (uint32_t)t;
+      br = (uint32_t)(t >> 63);
+    }
+    if (br) break;  // s < r
+#pragma unroll
+    for (int i = 0; i < 8; i++) s[i] = d[i];
+  }
+}
+
+__device__ __forceinline__ uint32_t get_bits(const uint32_t* s, int lo, int c) {
+  int k = lo >> 5, off = lo & 31;
+  uint32_t v = s[k] >> off;
+  if (off + c > 32 && k + 1 < 8) v |= s[k + 1] << (32 - off);
+  return v & ((1u << c) - 1u);
+}
+
+__global__ void k_digits(const uint32_t* __restrict__ scalars, uint32_t n, int c, int W, uint32_t* __restrict__ keys,
+                         uint32_t* __restrict__ vals, uint32_t* __restrict__ counts) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t s[8];
+  const uint4* q = reinterpret_cast<const uint4*>(scalars + 8ull * i);
+  uint4 a = q[0], b = q[1];
+  s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w;
+  s[4] = b.x; s[5] = b.y; s[6] = b.z; s[7] = b.w;
+  reduce_mod_r(s);
+  const uint32_t B = 1u << (c - 1);
+  uint32_t carry = 0;
+  for (int w = 0; w < W; w++) {
+    uint32_t v = get_bits(s, w * c, c) + carry;
+    int32_t d;
+    if (v > B) {
+      d = (int32_t)v - (int32_t)(1u << c);
+      carry = 1;
+    } else {
+      d = (int32_t)v;
+      carry = 0;
+    }
+    uint32_t key = NOKEY, val = 0;
+    if (d != 0) {
+      uint32_t mag = d < 0 ? (uint32_t)(-d) : (uint32_t)d;
+      key = (uint32_t)w * B + mag - 1;
+      val = i | (d < 0 ? 0x80000000u : 0u);
+      atomicAdd(&counts[key], 1u);
+    }
+    keys[(uint64_t)w * n + i] = key;
+    vals[(uint64_t)w * n + i] = val;
+  }
+}
+
+// exclusive scan, three kernels: 2048 elements per block
+static constexpr int SCAN_T = 1024;
+__global__ void k_scan_local(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t* __restrict__ bsum,
+                             uint32_t n) {
+  __shared__ uint32_t sh[2 * SCAN_T];
+  uint32_t base = blockIdx.x * 2 * SCAN_T;
+  uint32_t t = threadIdx.x;
+  uint32_t i0 = base + 2 * t, i1 = i0 + 1;
+  uint32_t a = i0 < n ? in[i0] : 0, b = i1 < n ? in[i1] : 0;
+  sh[2 * t] = a;
+  sh[2 * t + 1] = a + b;
+  __syncthreads();
+  // inclusive scan of the pair sums (Hillis-Steele over SCAN_T)
+  uint32_t v = sh[2 * t + 1];
+  for (int d = 1; d < SCAN_T; d <<= 1) {
+    uint32_t x = t >= (uint32_t)d ? sh[2 * (t - d) + 1] : 0;
+    __syncthreads();
+    v += x;
+    sh[2 * t + 1] = v;
+    __syncthreads();
+  }
+  uint32_t excl = v - (a + b);
+  if (i0 < n) out[i0] = excl;
+  if (i1 < n) out[i1] = excl + a;
+  if (t == SCAN_T - 1) bsum[blockIdx.x] = v;
+}
+
+// single block: exclusive scan of nblocks block sums in place; writes the total to *total
+__global__ void k_scan_top(uint32_t* __restrict__ bsum, uint32_t nblocks, uint32_t* __restrict__ total) {
+  __shared__ uint32_t sh[SCAN_T];
+  uint32_t t = threadIdx.x;
+  uint32_t per = (nblocks + SCAN_T - 1) / SCAN_T;
+  uint32_t lo = t * per, hi = min(lo + per, nblocks);
+  uint32_t s = 0;
+  for (uint32_t i = lo; i < hi; i++) s += bsum[i];
+  sh[t] = s;
+  __syncthreads();
+  uint32_t v = s;
+  for (int d = 1; d < SCAN_T; d <<= 1) {
+    uint32_t x = t >= (uint32_t)d ? sh[t - d] : 0;
+    __syncthreads();
+    v += x;
+    sh[t] = v;
+    __syncthreads();
+  }
+  uint32_t run = v - s;
+  for (uint32_t i = lo; i < hi; i++) {
+    uint32_t x = bsum[i];
+    bsum[i] = run;
+    run += x;
+  }
+  if (t == SCAN_T - 1) *total = v;
+}
+
+__global__ void k_scan_add(uint32_t* __restrict__ out, const uint32_t* __restrict__ bsum, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += bsum[i / (2 * SCAN_T)];
+}
+
+__global__ void k_copy_u32(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i];
+}
+
+__global__ void k_zero_u32(uint32_t* __restrict__ out, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = 0;
+}
+
+__global__ void k_scatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, uint64_t total,
+                          uint32_t* __restrict__ cursor, uint32_t* __restrict__ skeys, uint32_t* __restrict__ svals) {
+  uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  uint32_t key = keys[e];
+  if (key == NOKEY) return;
+  uint32_t pos = atomicAdd(&cursor[key], 1u);
+  skeys[pos] = key;
+  svals[pos] = vals[e];
+}
+
+__global__ void k_pack16(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint64_t nout) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nout) out[i] = (in[2 * i] & 0xFFFFu) | (in[2 * i + 1] << 16);
+}
+__global__ void k_unpack16(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint64_t nin) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nin) {
+    uint32_t v = in[i];
+    out[2 * i] = v & 0xFFFFu;
+    out[2 * i + 1] = v >> 16;
+  }
+}
+
+void launch_pack16(const uint32_t* in16, uint32_t* out32, uint64_t out_words, hipStream_t s) {
+  if (!out_words) return;
+  k_pack16<<<(unsigned)((out_words + 255) / 256), 256, 0, s>>>(in16, out32, out_words);
+  BG_HIP(hipGetLastError());
+}
+void launch_unpack16(const uint32_t* in32, uint32_t* out16, uint64_t in_words, hipStream_t s) {
+  if (!in_words) return;
+  k_unpack16<<<(unsigned)((in_words + 255) / 256), 256, 0, s>>>(in32, out16, in_words);
+  BG_HIP(hipGetLastError());
+}
+
+static unsigned blocks_for(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+uint32_t msm_scan_blocks(uint32_t n) { return blocks_for(n, 2 * SCAN_T); }
+
+void launch_zero_u32(uint32_t* p, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  k_zero_u32<<<blocks_for(n, 256), 256, 0, s>>>(p, n);
+  BG_HIP(hipGetLastError());
+}
+
+// digits + counting sort of (bucket, point) pairs by bucket
+void msm_sort(const uint32_t* d_scalars, uint64_t n, const MsmPlan& p, hipStream_t s, int group, uint32_t* keys,
+              uint32_t* vals, uint32_t* counts, uint32_t* offsets, uint32_t* bsum, uint32_t* cursor, uint32_t* skeys,
+              uint32_t* svals) {
+  uint32_t nblk = msm_scan_blocks(p.nb);
+  {
+      ScopedKTime kt(group == 1 ? "msm_g1_digits" : "msm_g2_digits", s);
+      k_zero_u32<<<blocks_for(p.nb + 1, 256), 256, 0, s>>>(counts, (uint64_t)p.nb + 1);
+      k_digits<<<blocks_for(n, 256), 256, 0, s>>>(d_scalars, (uint32_t)n, p.c, p.W, keys, vals, counts);
+      BG_HIP(hipGetLastError());
+    }
+    {
+      ScopedKTime kt(group == 1 ? "msm_g1_sort" : "msm_g2_sort", s);
+      k_scan_local<<<nblk, SCAN_T, 0, s>>>(counts, offsets, bsum, p.nb);
+      k_scan_top<<<1, SCAN_T, 0, s>>>(bsum, nblk, offsets + p.nb);
+      k_scan_add<<<blocks_for(p.nb, 256), 256, 0, s>>>(offsets, bsum, p.nb);
+      k_copy_u32<<<blocks_for(p.nb, 256), 256, 0, s>>>(offsets, cursor, p.nb);
+      k_scatter<<<blocks_for(p.entries, 256), 256, 0, s>>>(keys, vals, p.entries, cursor, skeys, svals);
+      BG_HIP(hipGetLastError());
+    }
+}
+
+void msm_enqueue(int group, const uint32_t* d_points, const uint32_t* d_scalars, uint64_t n, int window_bits,
+                 hipStream_t s, MsmWork& w) {
+  if (group == 1)
+    enqueue_g1(d_points, d_scalars, n, window_bits, s, w);
+  else
+    enqueue_g2(d_points, d_scalars, n, window_bits, s, w);
+}
+
+template <class E>
+static bgh::Pt<E> host_finish(const MsmPlan& p, const uint32_t* hout) {
+  using Pt = bgh::Pt<E>;
+  const int xw = 4 * bgh::words<E>();
+  int log2M = 0;
+  while ((1u << log2M) < p.m * p.K) log2M++;
+  std::vector<Pt> wsum(p.W, Pt::inf());
+  auto one_window = [&](int w) {
+    const uint32_t* base = hout + (uint64_t)2 * p.H * xw * w;
+    Pt sumP = Pt::inf(), run = Pt::inf(), wq = Pt::inf();
+    for (int h = (int)p.H - 1; h >= 0; h--) {
+      sumP = sumP.add(bgh::load_xyzz<E>(base + (uint64_t)xw * (2 * h)));
+      if (h > 0) {
+        run = run.add(bgh::load_xyzz<E>(base + (uint64_t)xw * (2 * h + 1)));
+        wq = wq.add(run);
+      }
+    }
+    wsum[w] = sumP.add(wq.dbl_n(log2M));
+  };
+  int nth = std::min<int>(p.W, (int)std::max(1u, std::thread::hardware_concurrency()));
+  if (p.H * p.W < 64 || nth <= 1) {
+    for (int w = 0; w < p.W; w++) one_window(w);
+  } else {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nth; t++)
+      th.emplace_back([&, t] {
+        for (int w = t; w < p.W; w += nth) one_window(w);
+      });
+    for (auto& x : th) x.join();
+  }
+  Pt acc = Pt::inf();
+  for (int w = p.W - 1; w >= 0; w--) acc = acc.dbl_n(p.c).add(wsum[w]);
+  return acc;
+}
+
+void msm_finish(MsmWork& w, uint32_t* out_affine, uint32_t* xyzz_out) {
+  BG_HIP(hipEventSynchronize(w.done));
+  const uint32_t* hout = static_cast<const uint32_t*>(w.host_out.ptr);
+  if (w.group == 1) {
+    auto r = host_finish<bgh::Fq>(w.plan, hout);
+    if (out_affine) bgh::store_affine(r, out_affine);
+    if (xyzz_out) {
+      r.X.to_u32(xyzz_out); r.Y.to_u32(xyzz_out + 8); r.ZZ.to_u32(xyzz_out + 16); r.ZZZ.to_u32(xyzz_out + 24);
+    }
+  } else {
+    auto r = host_finish<bgh::Fq2>(w.plan, hout);
+    if (out_affine) bgh::store_affine(r, out_affine);
+    if (xyzz_out) {
+      r.X.to_u32(xyzz_out); r.Y.to_u32(xyzz_out + 16); r.ZZ.to_u32(xyzz_out + 32); r.ZZZ.to_u32(xyzz_out + 48);
+    }
+  }
+}
+
+}  // namespace bg
+
+using namespace bg;
+
+// One-shot MSM through the C ABI (host or device buffers, either encoding).
+static int msm_api(int group, const uint32_t* points, const uint32_t* scalars, uint64_t n, uint32_t* out, int flags,
+                   int window_bits) {
+  return guard([&] {
+    BG_REQUIRE(out && (n == 0 || (points && scalars)), BG_ERR_ARG, "bg_msm: null pointer");
+    BG_REQUIRE(window_bits == 0 || (window_bits >= 2 && window_bits <= 22), BG_ERR_ARG,
+               "bg_msm: window_bits %d out of range (0 or 2..22)", window_bits);
+    bool words16 = (flags & 1) == BG_FMT_WORDS16;
+    bool dev = (flags & BG_MEM_DEVICE) != 0;
+    BG_REQUIRE(!(dev && words16), BG_ERR_ARG, "bg_msm: device buffers must be packed");
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> lk(c.mu);
+    const uint64_t ew = 8;                    // packed words per field element
+    const uint64_t pw = (group == 1 ? 2 : 4) * ew;  // packed words per affine point
+    const uint32_t* dp = points;
+    const uint32_t* ds = scalars;
+    if (!dev && n > 0) {
+      uint32_t* bp = c.scratch[10].as<uint32_t>(n * pw);
+      uint32_t* bs = c.scratch[11].as<uint32_t>(n * ew);
+      if (words16) {
+        uint32_t* rp = c.scratch[12].as<uint32_t>(2 * n * pw);
+        uint32_t* rs = c.scratch[13].as<uint32_t>(2 * n * ew);
+        BG_HIP(hipMemcpyAsync(rp, points, 2 * n * pw * 4, hipMemcpyHostToDevice, c.stream));
+        BG_HIP(hipMemcpyAsync(rs, scalars, 2 * n * ew * 4, hipMemcpyHostToDevice, c.stream));
+        launch_pack16(rp, bp, n * pw, c.stream);
+        launch_pack16(rs, bs, n * ew, c.stream);
+      } else {
+        BG_HIP(hipMemcpyAsync(bp, points, n * pw * 4, hipMemcpyHostToDevice, c.stream));
+        BG_HIP(hipMemcpyAsync(bs, scalars, n * ew * 4, hipMemcpyHostToDevice, c.stream));
+      }
+      dp = bp;
+      ds = bs;
+    }
+    static MsmWork work[2];
+    MsmWork& w = work[group - 1];
+    msm_enqueue(group, dp, ds, n, window_bits, c.stream, w);
+    uint32_t res[32];
+    msm_finish(w, res);
+    if (dev) {
+      BG_HIP(hipMemcpy(out, res, pw * 4, hipMemcpyHostToDevice));
+    } else if (words16) {
+      for (uint64_t i = 0; i < pw; i++) {
+        out[2 * i] = res[i] & 0xFFFFu;
+        out[2 * i + 1] = res[i] >> 16;
+      }
+    } else {
+      memcpy(out, res, pw * 4);
+    }
+  });
+}
+
+extern "C" int bg_msm_g1(const uint32_t* points, const uint32_t* scalars, uint64_t n, uint32_t* out, int flags,
+                         int window_bits) {
+  return msm_api(1, points, scalars, n, out, flags, window_bits);
+}
+
+extern "C" int bg_msm_g2(const uint32_t* points, const uint32_t* scalars, uint64_t n, uint32_t* out, int flags,
+                         int window_bits) {
+  return msm_api(2, points, scalars, n, out, flags, window_bits);
+}

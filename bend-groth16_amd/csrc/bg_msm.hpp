@@ -1,0 +1,69 @@
+// Pippenger MSM driver interface (device pipeline in bg_msm.hip).
+//
+// Replaces Msm.g1 / Msm.g2 (bend2/src/msm.bend:293-298, :525-530).  The reference uses
+// unsigned c-bit windows with private per-leaf bucket arrays; here the scalars are cut
+// into signed c-bit digits (2^(c-1) buckets per window), the (bucket, point) pairs are
+// counting-sorted on the device, each thread accumulates a fixed-length slice of the
+// sorted list (exact merging of buckets that straddle slices), and the per-window
+// weighted bucket sums run as a two-level parallel reduction.  The last few points per
+// window and the window combination run on the host.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "bg_runtime.hpp"
+
+namespace bg {
+
+struct MsmPlan {
+  int c = 0;         // window width (bits)
+  int W = 0;         // number of windows, W*c >= 255
+  uint32_t B = 0;    // buckets per window, 2^(c-1)
+  uint32_t nb = 0;   // W*B
+  uint32_t m = 0;    // buckets per level-1 thread
+  uint32_t G = 0;    // level-1 groups per window
+  uint32_t K = 0;    // level-2 block size
+  uint32_t H = 0;    // level-2 outputs per window
+  uint32_t L = 0;    // sorted entries per accumulation thread
+  uint64_t n = 0;
+  uint64_t entries = 0;  // W*n
+  uint32_t acc_threads = 0;
+};
+
+MsmPlan msm_plan(uint64_t n, int group, int window_bits);
+
+// Work buffers of one in-flight MSM (reused across calls).
+struct MsmWork {
+  DevBuf keys, vals, counts, skeys, svals, buckets, pkeys, pflags, ppts, lvlS, lvlT, outPQ, scan_tmp, red_scratch;
+  HostBuf host_out;
+  MsmPlan plan;
+  int group = 1;
+  hipEvent_t done = nullptr;
+};
+
+// Enqueue the device part of an MSM on stream s: points (packed affine, device),
+// scalars (packed canonical, device), n.  Results land in w.host_out (pinned) when the
+// recorded event completes.
+void msm_enqueue(int group, const uint32_t* d_points, const uint32_t* d_scalars, uint64_t n, int window_bits,
+                 hipStream_t s, MsmWork& w);
+
+// Waits for the MSM and finishes it on the host: out = affine result (packed words:
+// G1 16, G2 32).  If xyzz_out is non-null, also writes the XYZZ sum (G1 32, G2 64 words).
+void msm_finish(MsmWork& w, uint32_t* out_affine, uint32_t* xyzz_out = nullptr);
+
+// Device kernels shared with the prover: FORMAT.md 16-bit words -> packed words.
+void launch_pack16(const uint32_t* in16, uint32_t* out32, uint64_t out_words, hipStream_t s);
+void launch_unpack16(const uint32_t* in32, uint32_t* out16, uint64_t in_words, hipStream_t s);
+
+
+// pipeline pieces shared by the per-group translation units
+uint32_t msm_scan_blocks(uint32_t n);
+void launch_zero_u32(uint32_t* p, uint64_t n, hipStream_t s);
+void msm_sort(const uint32_t* d_scalars, uint64_t n, const MsmPlan& p, hipStream_t s, int group, uint32_t* keys,
+              uint32_t* vals, uint32_t* counts, uint32_t* offsets, uint32_t* bsum, uint32_t* cursor, uint32_t* skeys,
+              uint32_t* svals);
+void enqueue_g1(const uint32_t* d_points, const uint32_t* d_scalars, uint64_t n, int window_bits, hipStream_t s,
+                MsmWork& w);
+void enqueue_g2(const uint32_t* d_points, const uint32_t* d_scalars, uint64_t n, int window_bits, hipStream_t s,
+                MsmWork& w);
+
+}  // namespace bg

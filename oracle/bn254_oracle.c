@@ -861,3 +861,108 @@ int or_prove(const u32 *hdr,
     free(ea); free(eb); free(ec); free(h); free(zc);
     return 0;
 }
+
+/* ------------------------------------------------------------ fixed-base batches
+ * out[i] = k_i * G (affine, Montgomery) for the standard generators, via 8-bit windowed
+ * tables T[w][d] = d * 2^(8w) * G and one batched inversion per chunk (Montgomery's
+ * trick).  Used to build large MSM inputs with known discrete logs (P_i = k_i G), so
+ * that MSM = (sum a_i k_i mod r) G can be checked with a single scalar multiplication. */
+static void g1_gen(g1a_t *g) { memcpy(g->x, FQ.one, 32); f_add(&FQ, g->y, FQ.one, FQ.one); }
+static void g2_gen(g2a_t *g) {
+    /* standard BN254 G2 generator (x = x0 + x1 u, y = y0 + y1 u), canonical words */
+    static const u64 X0[4] = {0x46debd5cd992f6edULL, 0x674322d4f75edaddULL, 0x426a00665e5c4479ULL, 0x1800deef121f1e76ULL};
+    static const u64 X1[4] = {0x97e485b7aef312c2ULL, 0xf1aa493335a9e712ULL, 0x7260bfb731fb5d25ULL, 0x198e9393920d483aULL};
+    static const u64 Y0[4] = {0x4ce6cc0166fa7daaULL, 0xe3d1e7690c43d37bULL, 0x4aab71808dcb408fULL, 0x12c85ea5db8c6debULL};
+    static const u64 Y1[4] = {0x55acdadcd122975bULL, 0xbc4b313370b38ef3ULL, 0xec9e99ad690c3395ULL, 0x090689d0585ff075ULL};
+    f_to_mont(&FQ, g->x->c0, X0); f_to_mont(&FQ, g->x->c1, X1);
+    f_to_mont(&FQ, g->y->c0, Y0); f_to_mont(&FQ, g->y->c1, Y1);
+}
+
+#define FIXED_BASE_IMPL(G, J, A, E, EINV, EMUL, ESQR, GENF)                                 \
+void or_##G##_gen_mul_batch(u64 *out, const u64 *ks, u64 n) {                              \
+    or_init_once();                                                                        \
+    A gen; GENF(&gen);                                                                     \
+    J *tab = (J *)malloc(sizeof(J) * 32 * 256);                                            \
+    J base; G##_from_affine(&base, &gen);                                                  \
+    for (int w = 0; w < 32; w++) {                                                         \
+        G##_infinity(&tab[w * 256]);                                                       \
+        for (int d = 1; d < 256; d++) G##_add(&tab[w * 256 + d], &tab[w * 256 + d - 1], &base); \
+        for (int k = 0; k < 8; k++) G##_double(&base, &base);                              \
+    }                                                                                      \
+    A *out_a = (A *)out;                                                                   \
+    _Pragma("omp parallel")                                                                \
+    {                                                                                      \
+        const u64 CH = 256;                                                                \
+        J *acc = (J *)malloc(sizeof(J) * CH);                                              \
+        E *pre = (E *)malloc(sizeof(E) * CH);                                              \
+        _Pragma("omp for schedule(dynamic, 1)")                                            \
+        for (u64 c0 = 0; c0 < n; c0 += CH) {                                               \
+            u64 m = n - c0 < CH ? n - c0 : CH;                                             \
+            for (u64 i = 0; i < m; i++) {                                                  \
+                const u64 *k = ks + 4 * (c0 + i);                                          \
+                G##_infinity(&acc[i]);                                                     \
+                for (int w = 0; w < 32; w++) {                                             \
+                    unsigned d = (unsigned)((k[w / 8] >> (8 * (w % 8))) & 0xFF);           \
+                    if (d) G##_add(&acc[i], &acc[i], &tab[w * 256 + d]);                   \
+                }                                                                          \
+            }                                                                              \
+            /* batched inversion of the z coordinates (zero z -> infinity) */              \
+            E run; memset(&run, 0, sizeof run); E one_; memset(&one_, 0, sizeof one_);      \
+            memcpy(&one_, FQ.one, 32); run = one_;                                         \
+            for (u64 i = 0; i < m; i++) {                                                  \
+                pre[i] = run;                                                              \
+                if (!G##_is_inf(&acc[i])) EMUL(&run, &run, (const E *)acc[i].z);           \
+            }                                                                              \
+            E inv; EINV(&inv, &run);                                                       \
+            for (u64 i = m; i-- > 0;) {                                                    \
+                A *o = &out_a[c0 + i];                                                     \
+                if (G##_is_inf(&acc[i])) { memset(o, 0, sizeof *o); continue; }            \
+                E zi, zi2, zi3;                                                            \
+                EMUL(&zi, &inv, &pre[i]);                                                  \
+                EMUL(&inv, &inv, (const E *)acc[i].z);                                     \
+                ESQR(&zi2, &zi); EMUL(&zi3, &zi2, &zi);                                    \
+                EMUL((E *)o->x, (const E *)acc[i].x, &zi2);                                \
+                EMUL((E *)o->y, (const E *)acc[i].y, &zi3);                                \
+            }                                                                              \
+        }                                                                                  \
+        free(acc); free(pre);                                                              \
+    }                                                                                      \
+    free(tab);                                                                             \
+}
+
+typedef struct { u64 v[4]; } fqe_t;
+static void fqe_mul(fqe_t *r, const fqe_t *a, const fqe_t *b) { f_mul(&FQ, r->v, a->v, b->v); }
+static void fqe_sqr(fqe_t *r, const fqe_t *a) { f_mul(&FQ, r->v, a->v, a->v); }
+static void fqe_inv(fqe_t *r, const fqe_t *a) { f_inv(&FQ, r->v, a->v); }
+static void fq2e_mul(fq2_t *r, const fq2_t *a, const fq2_t *b) { fq2_mul(r, a, b); }
+static void fq2e_sqr(fq2_t *r, const fq2_t *a) { fq2_sqr(r, a); }
+static void fq2e_inv(fq2_t *r, const fq2_t *a) { fq2_inv(r, a); }
+
+FIXED_BASE_IMPL(g1, g1j_t, g1a_t, fqe_t, fqe_inv, fqe_mul, fqe_sqr, g1_gen)
+FIXED_BASE_IMPL(g2, g2j_t, g2a_t, fq2_t, fq2e_inv, fq2e_mul, fq2e_sqr, g2_gen)
+
+/* sum_i a_i * k_i mod r for canonical a_i, k_i (4 words each) -> canonical (4 words) */
+void or_fr_dot(u64 *out, const u64 *a, const u64 *k, u64 n) {
+    or_init_once();
+    u64 acc[4] = {0, 0, 0, 0};
+    #pragma omp parallel
+    {
+        u64 loc[4] = {0, 0, 0, 0};
+        #pragma omp for schedule(static)
+        for (u64 i = 0; i < n; i++) {
+            u64 x[4], y[4], t[4];
+            /* a, k canonical: (a R)(k R) R^-1 = a k R, so convert one operand */
+            f_to_mont(&FR, x, a + 4 * i);
+            memcpy(y, k + 4 * i, 32);
+            if (geq4(y, FR.p)) { /* reduce scalars >= r (at most 5 subtractions) */
+                while (geq4(y, FR.p)) sub4(y, y, FR.p);
+            }
+            if (geq4(x, FR.p)) sub4(x, x, FR.p);
+            f_mul(&FR, t, x, y);   /* = a k (canonical) */
+            f_add(&FR, loc, loc, t);
+        }
+        #pragma omp critical
+        f_add(&FR, acc, acc, loc);
+    }
+    memcpy(out, acc, 32);
+}

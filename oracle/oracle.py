@@ -309,6 +309,8 @@ def lib():
             "or_qap_h": [P, P, P, P, ctypes.c_int],
             "or_prove": [P] * 23 + [P],
             "or_threads": [], "or_set_threads": [ctypes.c_int],
+            "or_g1_gen_mul_batch": [P, P, U64], "or_g2_gen_mul_batch": [P, P, U64],
+            "or_fr_dot": [P, P, P, U64],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -422,6 +424,44 @@ def prove(inst: dict):
     if rc != 0:
         raise ValueError(f"or_prove rejected the instance (rc={rc})")
     return A, B, C, t
+
+
+def g1_gen_mul_batch(ks: np.ndarray) -> np.ndarray:
+    """(n, 4) canonical scalars -> (n, 8) affine k_i * G1 (Montgomery u64)."""
+    ks = np.ascontiguousarray(ks, dtype=np.uint64)
+    out = np.zeros((ks.shape[0], 8), dtype=np.uint64)
+    lib().or_g1_gen_mul_batch(_ptr(out), _ptr(ks), ks.shape[0])
+    return out
+
+
+def g2_gen_mul_batch(ks: np.ndarray) -> np.ndarray:
+    ks = np.ascontiguousarray(ks, dtype=np.uint64)
+    out = np.zeros((ks.shape[0], 16), dtype=np.uint64)
+    lib().or_g2_gen_mul_batch(_ptr(out), _ptr(ks), ks.shape[0])
+    return out
+
+
+def fr_dot(a: np.ndarray, k: np.ndarray) -> int:
+    """sum_i a_i k_i mod r (canonical inputs, (n, 4) u64) -> int."""
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    k = np.ascontiguousarray(k, dtype=np.uint64)
+    out = np.zeros(4, dtype=np.uint64)
+    lib().or_fr_dot(_ptr(out), _ptr(a), _ptr(k), a.shape[0])
+    return from_words(out)
+
+
+def rand_scalars(rng: np.random.Generator, n: int, bound: int = P_R) -> np.ndarray:
+    """(n, 4) u64 uniform in [0, bound) (rejection on the top word)."""
+    out = rng.integers(0, 2**64, size=(n, 4), dtype=np.uint64)
+    top = bound >> 192
+    out[:, 3] = rng.integers(0, top + 1, size=n, dtype=np.uint64)
+    vals = out[:, 3] == top
+    if vals.any():  # values equal to the top word: redraw below the bound exactly
+        for i in np.nonzero(vals)[0]:
+            while from_words(out[i]) >= bound:
+                out[i] = rng.integers(0, 2**64, size=4, dtype=np.uint64)
+                out[i, 3] = rng.integers(0, top + 1, dtype=np.uint64)
+    return out
 
 
 def set_threads(n: int) -> None:

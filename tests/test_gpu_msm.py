@@ -1,0 +1,124 @@
+"""GPU parity of the Pippenger MSM (Msm.g1 / Msm.g2, bend2/src/msm.bend:293-298, :525-530)
+against the CPU oracle (brute-force-pinned Pippenger restatement) and, at large n, the
+closed form MSM = (sum a_i k_i mod r) G for bases P_i = k_i G.  Affine output is unique,
+so the comparison is bit-exact."""
+import numpy as np
+import pytest
+
+import bendg
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+R = O.P_R
+
+
+def _u32(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+def _mul_gen(group, k: int):
+    kk = np.array(O.to_words(k % R), dtype=np.uint64)
+    if group == 1:
+        out = np.zeros(8, dtype=np.uint64)
+        O.lib().or_g1_mul_aff(O._ptr(out), O._ptr(O.g1_pack([O.G1_GEN])[0]), O._ptr(kk))
+    else:
+        out = np.zeros(16, dtype=np.uint64)
+        O.lib().or_g2_mul_aff(O._ptr(out), O._ptr(O.g2_pack([O.G2_GEN])[0]), O._ptr(kk))
+    return out
+
+
+def _bases(group, rng, n):
+    ks = O.rand_scalars(rng, n)
+    pts = O.g1_gen_mul_batch(ks) if group == 1 else O.g2_gen_mul_batch(ks)
+    return pts, ks
+
+
+def _msm(group, pts, sc, **kw):
+    f = bendg.msm_g1 if group == 1 else bendg.msm_g2
+    return f(_u32(pts), _u32(sc), **kw).view(np.uint64)
+
+
+def _oracle(group, pts, sc):
+    return O.msm_g1(pts, sc) if group == 1 else O.msm_g2(pts, sc)
+
+
+@pytest.mark.parametrize("group", [1, 2], ids=["G1", "G2"])
+@pytest.mark.parametrize("n", [1, 2, 3, 8, 64, 1024])
+def test_msm_small_vs_oracle(bg, group, n):
+    rng = np.random.default_rng(100 + n + 7 * group)
+    pts, _ = _bases(group, rng, n)
+    sc = O.rand_scalars(rng, n)
+    print(f"msm G{group} n={n}", flush=True)
+    assert np.array_equal(_msm(group, pts, sc), _oracle(group, pts, sc))
+
+
+@pytest.mark.parametrize("group", [1, 2], ids=["G1", "G2"])
+def test_msm_edge_cases(bg, group):
+    """zero / one / r-1 / >= r scalars, equal scalars (one heavy bucket), infinity bases,
+    repeated and opposite bases (bucket doubling and cancellation)."""
+    rng = np.random.default_rng(5 + group)
+    n = 600
+    pts, _ = _bases(group, rng, n)
+    sc = O.rand_scalars(rng, n)
+    sc[0] = 0
+    sc[1] = O.to_words(1)
+    sc[2] = O.to_words(R - 1)
+    sc[3] = O.to_words((1 << 256) - 1)          # reduced mod r by the MSM
+    sc[4] = O.to_words(R)                        # == 0 mod r
+    sc[10:300] = sc[9]                           # 290 equal scalars -> heavy buckets
+    pts[20] = 0                                  # infinity bases
+    pts[21] = 0
+    pts[30] = pts[31]                            # same base twice
+    sc[30] = sc[31]
+    pts[40] = pts[41]                            # P and -P with equal scalars cancel
+    half = pts.shape[1] // 2
+    negy = O.ints_to_array([(O.P_Q - O.unmont(v, O.P_Q)) % O.P_Q for v in
+                            O.array_to_ints(pts[41, half:].reshape(-1, 4), None)], O.P_Q)
+    pts[40, half:] = negy.reshape(-1)
+    sc[40] = sc[41]
+    want_sc = sc.copy()
+    want_sc[3] = O.to_words(((1 << 256) - 1) % R)
+    want_sc[4] = 0
+    got = _msm(group, pts, sc)
+    assert np.array_equal(got, _oracle(group, pts, want_sc))
+
+
+@pytest.mark.parametrize("c", [2, 3, 5, 8, 11, 13, 16])
+def test_msm_window_widths(bg, c):
+    rng = np.random.default_rng(c)
+    n = 300
+    pts, _ = _bases(1, rng, n)
+    sc = O.rand_scalars(rng, n)
+    assert np.array_equal(_msm(1, pts, sc, window_bits=c), _oracle(1, pts, sc))
+
+
+def test_msm_all_zero_and_empty(bg):
+    rng = np.random.default_rng(9)
+    pts, _ = _bases(1, rng, 16)
+    sc = np.zeros((16, 4), dtype=np.uint64)
+    assert not _msm(1, pts, sc).any()
+    out = bendg.msm_g1(np.zeros((0, 16), np.uint32), np.zeros((0, 8), np.uint32))
+    assert not out.any()
+
+
+def test_msm_words16_encoding(bg):
+    rng = np.random.default_rng(12)
+    n = 100
+    pts, _ = _bases(1, rng, n)
+    sc = O.rand_scalars(rng, n)
+    p16 = bendg.unpack16(_u32(pts).reshape(n, 16))
+    s16 = bendg.unpack16(_u32(sc).reshape(n, 8))
+    got16 = bendg.msm_g1(p16, s16, fmt="words16")
+    assert np.array_equal(bendg.pack16(got16).view(np.uint64), _oracle(1, pts, sc))
+
+
+@pytest.mark.parametrize("group,log2n", [(1, 14), (1, 16), (1, 18), (2, 12), (2, 14)])
+def test_msm_closed_form(bg, group, log2n):
+    rng = np.random.default_rng(1000 + log2n + group)
+    n = 1 << log2n
+    pts, ks = _bases(group, rng, n)
+    sc = O.rand_scalars(rng, n)
+    want = _mul_gen(group, O.fr_dot(sc, ks))
+    print(f"msm closed form G{group} 2^{log2n}", flush=True)
+    assert np.array_equal(_msm(group, pts, sc), want)
