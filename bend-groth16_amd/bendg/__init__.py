@@ -73,6 +73,7 @@ def lib() -> ctypes.CDLL:
         "bg_field_op": [I, I, P, P, P, U64],
         "bg_curve_op": [I, I, P, P, P, U64],
         "bg_probe_rate": [I, U64, P],
+        "bg_msm_plan_info": [U64, I, I, P],
     }
     for name, args in sigs.items():
         fn = getattr(L, name, None)
@@ -226,6 +227,13 @@ def qap_h(a: np.ndarray, b: np.ndarray, c: np.ndarray, log2n: int, fmt: str = "p
     h = np.zeros_like(a)
     _check(lib().bg_qap_h(_p(a), _p(b), _p(c), log2n, _p(h), _fmt_flag(fmt)))
     return h
+
+
+def msm_plan_info(n: int, group: int = 1, window_bits: int = 0) -> dict:
+    info = np.zeros(8, dtype=np.uint64)
+    _check(lib().bg_msm_plan_info(n, group, window_bits, _p(info)))
+    keys = ("c", "windows", "buckets", "entries", "L", "acc_threads", "groups1", "outputs2")
+    return {k: int(v) for k, v in zip(keys, info)}
 
 
 # ---------------------------------------------------------------- timing

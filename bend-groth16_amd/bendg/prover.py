@@ -10,7 +10,7 @@ import numpy as np
 
 from . import FMT_PACKED, FMT_WORDS16, _check, _p, lib
 
-PHASES = ("upload", "evals", "qap", "msm_a", "msm_b1", "msm_b2", "msm_hl", "finish")
+PHASES = ("upload", "evals", "qap", "msm_a", "msm_b1", "msm_b2", "msm_h", "msm_l", "total")
 
 
 class ProvingKey:
@@ -34,6 +34,17 @@ class ProvingKey:
         _check(lib().bg_pk_device_bytes(self._h, _p(b)))
         return int(b[0])
 
+    def prove_device(self, witness_ptr: int, rs: np.ndarray):
+        """Proof with the packed witness already resident in HBM (device pointer);
+        rs is host (packed).  Returns (packed proof, phase times)."""
+        r = np.ascontiguousarray(rs, dtype=np.uint32).reshape(-1)
+        if r.size != 16:
+            raise ValueError("rs must hold two packed Fr elements")
+        out = np.zeros(64, dtype=np.uint32)
+        t = np.zeros(9, dtype=np.float64)
+        _check(lib().bg_prove(self._h, ctypes.c_void_p(witness_ptr), _p(r), _p(out), FMT_PACKED | 2, _p(t)))
+        return out, dict(zip(PHASES, (float(x) for x in t)))
+
     def prove(self, witness: np.ndarray, rs: np.ndarray, fmt: str = "words16"):
         """witness / rs in FORMAT.md words (``words16``) or packed u32; returns
         (proof uint32 array [A | B | C] in the same encoding, phase times dict)."""
@@ -46,7 +57,7 @@ class ProvingKey:
         if w.size != n_z * ew:
             raise ValueError(f"witness must hold {n_z} Fr elements")
         out = np.zeros(8 * ew, dtype=np.uint32)
-        t = np.zeros(8, dtype=np.float64)
+        t = np.zeros(9, dtype=np.float64)
         flag = FMT_WORDS16 if fmt == "words16" else FMT_PACKED
         _check(lib().bg_prove(self._h, _p(w), _p(r), _p(out), flag, _p(t)))
         return out, dict(zip(PHASES, (float(x) for x in t)))

@@ -5,6 +5,7 @@
 #pragma once
 #include <stdint.h>
 #include <string.h>
+#include <vector>
 
 namespace bgh {
 
@@ -201,6 +202,17 @@ struct Pt {
     E y3 = r * (qq - x3) - s1 * ppp;
     return {x3, y3, ZZ * q.ZZ * pp, ZZZ * q.ZZZ * ppp};
   }
+  // mixed add with an affine point (x, y) that is not infinity: madd-2008-s
+  Pt madd(const E& x, const E& y) const {
+    if (is_inf()) return {x, y, E::one(), E::one()};
+    E u2 = x * ZZ, s2 = y * ZZZ;
+    E p = u2 - X, r = s2 - Y;
+    if (p.is_zero()) return r.is_zero() ? from_affine(x, y).dbl() : inf();
+    E pp = p.sqr(), ppp = p * pp, qq = X * pp;
+    E x3 = r.sqr() - ppp - qq.dbl();
+    E y3 = r * (qq - x3) - Y * ppp;
+    return {x3, y3, ZZ * pp, ZZZ * ppp};
+  }
   Pt neg() const { return {X, -Y, ZZ, ZZZ}; }
   Pt dbl_n(int k) const {
     Pt r = *this;
@@ -255,6 +267,30 @@ void store_affine(const Pt<E>& p, uint32_t* w) {
   p.to_affine(x, y);
   x.to_u32(w);
   y.to_u32(w + k);
+}
+
+// Affine normalisation of a batch with one inversion (Montgomery's trick).
+template <class E>
+void batch_to_affine(const Pt<E>* pts, size_t n, E* xs, E* ys) {
+  std::vector<E> pre(n);
+  E run = E::one();
+  for (size_t i = 0; i < n; i++) {
+    pre[i] = run;
+    if (!pts[i].is_inf()) run = run * (pts[i].ZZ * pts[i].ZZZ);
+  }
+  E inv = run.inv();
+  for (size_t i = n; i-- > 0;) {
+    if (pts[i].is_inf()) {
+      xs[i] = E::zero();
+      ys[i] = E::zero();
+      continue;
+    }
+    E d = pts[i].ZZ * pts[i].ZZZ;
+    E t = inv * pre[i];
+    inv = inv * d;
+    xs[i] = pts[i].X * pts[i].ZZZ * t;
+    ys[i] = pts[i].Y * pts[i].ZZ * t;
+  }
 }
 
 }  // namespace bgh

@@ -117,24 +117,19 @@ __global__ void k_curve_op(int op, const uint32_t* p, const uint32_t* q, uint32_
 
 // ---------------------------------------------------------------- throughput probes
 __global__ void k_probe_mad(uint64_t iters, uint32_t seed, uint64_t* sink) {
+  // 16 independent chains of acc = lo(acc) * y + acc: one v_mad_u64_u32 per step
   uint32_t t = threadIdx.x + blockIdx.x * blockDim.x;
-  uint64_t acc[8];
-  uint32_t x[8];
+  uint64_t acc[16];
 #pragma unroll
-  for (int k = 0; k < 8; k++) {
-    acc[k] = t + k;
-    x[k] = seed * (k + 1) + t;
-  }
+  for (int k = 0; k < 16; k++) acc[k] = ((uint64_t)(seed * (k + 1)) << 32) | (t + k);
   uint32_t y = seed ^ t;
   for (uint64_t it = 0; it < iters; it++) {
 #pragma unroll
-    for (int k = 0; k < 8; k++) acc[k] = (uint64_t)x[k] * y + acc[k];
-#pragma unroll
-    for (int k = 0; k < 8; k++) x[k] ^= (uint32_t)acc[k];
+    for (int k = 0; k < 16; k++) acc[k] = (uint64_t)(uint32_t)acc[k] * y + acc[k];
   }
   uint64_t s = 0;
 #pragma unroll
-  for (int k = 0; k < 8; k++) s += acc[k];
+  for (int k = 0; k < 16; k++) s ^= acc[k];
   if (s == 0x1234567ull) sink[0] = s;
 }
 
@@ -279,7 +274,7 @@ extern "C" int bg_probe_rate(int kind, uint64_t iters, double* result) {
     BG_HIP(hipEventSynchronize(e1));
     float ms = 0;
     BG_HIP(hipEventElapsedTime(&ms, e0, e1));
-    double per_thread = kind == 0 ? 8.0 : (kind == 1 ? 4.0 : 1.0);
+    double per_thread = kind == 0 ? 16.0 : (kind == 1 ? 4.0 : 1.0);
     result[0] = (double)blocks * threads * per_thread * (double)iters / (ms * 1e-3);
     result[1] = ms;
     (void)hipEventDestroy(e0);

@@ -357,6 +357,21 @@ static int msm_api(int group, const uint32_t* points, const uint32_t* scalars, u
   });
 }
 
+extern "C" int bg_msm_plan_info(uint64_t n, int group, int window_bits, uint64_t* info8) {
+  return guard([&] {
+    BG_REQUIRE(info8 && (group == 1 || group == 2), BG_ERR_ARG, "bg_msm_plan_info: bad argument");
+    MsmPlan p = msm_plan(n, group, window_bits);
+    info8[0] = (uint64_t)p.c;
+    info8[1] = (uint64_t)p.W;
+    info8[2] = p.B;
+    info8[3] = p.entries;
+    info8[4] = p.L;
+    info8[5] = p.acc_threads;
+    info8[6] = (uint64_t)p.W * p.G;  // level-1 groups
+    info8[7] = (uint64_t)p.W * p.H;  // level-2 outputs (host tail points / 2)
+  });
+}
+
 extern "C" int bg_msm_g1(const uint32_t* points, const uint32_t* scalars, uint64_t n, uint32_t* out, int flags,
                          int window_bits) {
   return msm_api(1, points, scalars, n, out, flags, window_bits);

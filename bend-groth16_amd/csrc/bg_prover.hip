@@ -1,0 +1,393 @@
+// Resident-key Groth16 prover (C ABI: bg_pk_load, bg_pk_free, bg_pk_header,
+// bg_pk_device_bytes, bg_prove, bg_proof_format).
+//
+// Device-side counterpart of bend2/prove.bend's Pr.run (:129-162): the FORMAT.md key is
+// loaded once into HBM (packed 8 x u32 limbs), and each proof runs
+//   upload z -> R1CS evaluation vectors (groth16.bend:162-208, with the libsnark
+//   input-consistency rows) and canonical scalars (groth16.bend:210-219, 263-270)
+//   -> qap_h (ntt.bend:444-459) -> five MSMs in the reference's order and slots
+//   (prove.bend:64-116) -> proof combination on the host (groth16.bend:298-308).
+#include "bg_runtime.hpp"
+#include "bg_msm.hpp"
+#include "bg_ntt.hpp"
+#include "bg_field.hpp"
+#include "bg_host_ec.hpp"
+#include <chrono>
+#include <memory>
+#include <fstream>
+#include <string>
+#include <vector>
+
+namespace bg {
+
+// ------------------------------------------------------------------ FORMAT.md codec
+
+static std::vector<uint32_t> read_words(const std::string& path, size_t expect_words = 0, bool exact = true) {
+  std::ifstream f(path, std::ios::binary | std::ios::ate);
+  BG_REQUIRE(f.good(), BG_ERR_IO, "cannot open %s", path.c_str());
+  std::streamsize sz = f.tellg();
+  BG_REQUIRE(sz >= 0 && sz % 4 == 0, BG_ERR_IO, "%s: size %lld is not a multiple of 4", path.c_str(), (long long)sz);
+  if (expect_words)
+    BG_REQUIRE(exact ? (size_t)sz == expect_words * 4 : (size_t)sz >= expect_words * 4, BG_ERR_IO,
+               "%s: %lld bytes, expected %zu", path.c_str(), (long long)sz, expect_words * 4);
+  std::vector<uint32_t> w((size_t)sz / 4);
+  f.seekg(0);
+  f.read(reinterpret_cast<char*>(w.data()), sz);
+  BG_REQUIRE(f.good() || sz == 0, BG_ERR_IO, "%s: read failed", path.c_str());
+  return w;
+}
+
+// 16-bit-limb words -> packed words; rejects limbs >= 2^16 (bend2/tools/verify_bin.py:38)
+static void pack_words(const uint32_t* in16, uint32_t* out32, size_t nout, const char* what) {
+  uint32_t bad = 0;
+  for (size_t i = 0; i < nout; i++) {
+    uint32_t lo = in16[2 * i], hi = in16[2 * i + 1];
+    bad |= (lo | hi) >> 16;
+    out32[i] = lo | (hi << 16);
+  }
+  BG_REQUIRE(bad == 0, BG_ERR_IO, "%s: limb word >= 2^16", what);
+}
+
+static std::vector<uint32_t> packed_file(const std::string& path, size_t nwords16, const char* what) {
+  auto w = read_words(path, nwords16);
+  std::vector<uint32_t> out(nwords16 / 2);
+  pack_words(w.data(), out.data(), out.size(), what);
+  return out;
+}
+
+struct Csr {
+  uint32_t nnz = 0;
+  DevArray offs, cols, coef;
+};
+
+}  // namespace bg
+
+struct bg_pk {
+  uint32_t hdr[8];  // nc, ni, nw, lg, na, nb, nh, nl
+  bg::Csr A, B, C;
+  bg::DevArray pa, pb1, pb2, ph, pl;
+  uint32_t g1[3 * 16];  // alpha, beta, delta (packed affine)
+  uint32_t g2[2 * 32];  // beta2, delta2
+  // per-proof device work space
+  bg::DevArray z, zc, ea, eb, ec, h, tmp, zraw;
+  bg::MsmWork work[5];
+  bg::HostBuf stage;
+  uint64_t dev_bytes = 0;
+};
+
+namespace bg {
+
+static void upload_array(DevArray& d, const void* src, size_t bytes, uint64_t& total) {
+  d.alloc(bytes);
+  if (bytes) BG_HIP(hipMemcpy(d.ptr, src, bytes, hipMemcpyHostToDevice));
+  total += bytes;
+}
+
+// ------------------------------------------------------------------ device kernels
+
+// Evaluation vector of one matrix over the 2^lg domain: row i < nc is the CSR row dot z
+// (coefficient Montgomery-one skips the product, groth16.bend:108-116); for A the rows
+// nc .. nc+ni-1 are the instance values z[i] (libsnark input consistency); zero elsewhere.
+__global__ void k_evals(const uint32_t* __restrict__ offs, const uint32_t* __restrict__ cols,
+                        const uint32_t* __restrict__ coef, const uint32_t* __restrict__ z, uint32_t nc, uint32_t ni,
+                        int with_inputs, uint64_t d, uint32_t* __restrict__ out) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d) return;
+  Fp acc = Fr::zero();
+  if (i < nc) {
+    const Fp one = Fr::one();
+    for (uint32_t e = offs[i]; e < offs[i + 1]; e++) {
+      Fp cf = load_fp(coef + 8ull * e);
+      Fp zv = load_fp(z + 8ull * cols[e]);
+      acc = Fr::add(acc, Fr::eq(cf, one) ? zv : Fr::mul(cf, zv));
+    }
+  } else if (with_inputs && i < (uint64_t)nc + ni) {
+    acc = load_fp(z + 8 * (i - nc));
+  }
+  store_fp(out + 8 * i, acc);
+}
+
+__global__ void k_from_mont(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) store_fp(out + 8 * i, Fr::from_mont(load_fp(in + 8 * i)));
+}
+
+static unsigned nb(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+// ------------------------------------------------------------------ host combination
+
+
+static void canon_words(const uint32_t* mont_fr, uint64_t out[4]) {
+  bgh::Fr x = bgh::Fr::from_u32(mont_fr).from_mont();
+  memcpy(out, x.v, 32);
+}
+
+}  // namespace bg
+
+using namespace bg;
+
+// host side of a CSR file, validated
+struct HostCsr {
+  uint32_t nnz = 0;
+  std::vector<uint32_t> offs, cols, coef;
+};
+
+static HostCsr read_csr(const std::string& path, uint32_t nc, uint64_t nz) {
+  auto w = read_words(path);
+  BG_REQUIRE(w.size() >= (size_t)nc + 2, BG_ERR_IO, "%s: truncated", path.c_str());
+  HostCsr h;
+  h.nnz = w[0];
+  size_t expect = 1 + (size_t)nc + 1 + 17ull * h.nnz;
+  BG_REQUIRE(w.size() == expect, BG_ERR_IO, "%s: %zu words, expected %zu for nnz=%u", path.c_str(), w.size(), expect,
+             h.nnz);
+  h.offs.assign(w.begin() + 1, w.begin() + 2 + nc);
+  BG_REQUIRE(h.offs[0] == 0 && h.offs[nc] == h.nnz, BG_ERR_IO, "%s: bad row offsets", path.c_str());
+  for (uint32_t i = 0; i < nc; i++)
+    BG_REQUIRE(h.offs[i] <= h.offs[i + 1], BG_ERR_IO, "%s: row offsets decrease", path.c_str());
+  h.cols.resize(h.nnz);
+  h.coef.resize(8ull * h.nnz);
+  const uint32_t* e = w.data() + 2 + nc;
+  for (uint32_t k = 0; k < h.nnz; k++) {
+    h.cols[k] = e[17ull * k];
+    BG_REQUIRE(h.cols[k] < nz, BG_ERR_IO, "%s: column %u out of range (%llu wires)", path.c_str(), h.cols[k],
+               (unsigned long long)nz);
+    pack_words(e + 17ull * k + 1, &h.coef[8ull * k], 8, path.c_str());
+  }
+  return h;
+}
+
+extern "C" int bg_pk_load(const char* dir, bg_pk** out) {
+  return guard([&] {
+    BG_REQUIRE(dir && out, BG_ERR_ARG, "bg_pk_load: null argument");
+    *out = nullptr;
+    std::string d(dir);
+    // 1. read and validate every file on the host (bend2/tools/verify_bin.py:24-64 checks)
+    auto hdrw = read_words(d + "/header.bin", 8);
+    auto pk = std::make_unique<bg_pk>();
+    memcpy(pk->hdr, hdrw.data(), 32);
+    uint32_t nc = pk->hdr[0], ni = pk->hdr[1], nw = pk->hdr[2], lg = pk->hdr[3];
+    uint32_t na = pk->hdr[4], nbq = pk->hdr[5], nh = pk->hdr[6], nl = pk->hdr[7];
+    BG_REQUIRE(lg >= 1 && lg <= 28, BG_ERR_IO, "%s/header.bin: domain_log2 %u out of range", dir, lg);
+    uint64_t dsz = 1ull << lg;
+    uint64_t nz = (uint64_t)ni + nw;
+    BG_REQUIRE(ni >= 1 && (uint64_t)nc + ni <= dsz && na <= nz && nbq <= nz && nh <= dsz && (uint64_t)nl + ni <= nz,
+               BG_ERR_IO, "%s/header.bin: inconsistent sizes", dir);
+    HostCsr ca = read_csr(d + "/r1cs_a.bin", nc, nz);
+    HostCsr cb = read_csr(d + "/r1cs_b.bin", nc, nz);
+    HostCsr cc = read_csr(d + "/r1cs_c.bin", nc, nz);
+    auto g1 = packed_file(d + "/pk_g1.bin", 3 * 32, "pk_g1.bin");
+    auto g2 = packed_file(d + "/pk_g2.bin", 2 * 64, "pk_g2.bin");
+    memcpy(pk->g1, g1.data(), sizeof pk->g1);
+    memcpy(pk->g2, g2.data(), sizeof pk->g2);
+    struct Q {
+      const char* name;
+      uint64_t n;
+      int w16;
+      DevArray* dst;
+      std::vector<uint32_t> host;
+    } qs[] = {{"pk_a.bin", na, 32, &pk->pa, {}},
+              {"pk_b1.bin", nbq, 32, &pk->pb1, {}},
+              {"pk_b2.bin", nbq, 64, &pk->pb2, {}},
+              {"pk_h.bin", nh, 32, &pk->ph, {}},
+              {"pk_l.bin", nl, 32, &pk->pl, {}}};
+    for (auto& q : qs) q.host = packed_file(d + "/" + q.name, q.n * q.w16, q.name);
+
+    // 2. upload into HBM
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> lk(c.mu);
+    uint64_t total = 0;
+    for (auto pr : {std::make_pair(&ca, &pk->A), std::make_pair(&cb, &pk->B), std::make_pair(&cc, &pk->C)}) {
+      pr.second->nnz = pr.first->nnz;
+      upload_array(pr.second->offs, pr.first->offs.data(), 4ull * (nc + 1), total);
+      upload_array(pr.second->cols, pr.first->cols.data(), 4ull * pr.first->nnz, total);
+      upload_array(pr.second->coef, pr.first->coef.data(), 32ull * pr.first->nnz, total);
+    }
+    for (auto& q : qs) upload_array(*q.dst, q.host.data(), q.host.size() * 4, total);
+    // per-proof work arrays
+    pk->z.alloc(32 * nz);
+    pk->zc.alloc(32 * nz);
+    pk->zraw.alloc(64 * nz);
+    pk->ea.alloc(32 * dsz);
+    pk->eb.alloc(32 * dsz);
+    pk->ec.alloc(32 * dsz);
+    pk->h.alloc(32 * dsz);
+    pk->tmp.alloc(32 * dsz);
+    total += 32 * nz * 2 + 64 * nz + 32 * dsz * 5;
+    pk->dev_bytes = total;
+    (void)ntt_domain((int)lg);  // twiddle / scale tables resident with the key
+    *out = pk.release();
+  });
+}
+
+extern "C" int bg_pk_free(bg_pk* pk) {
+  return guard([&] {
+    if (!pk) return;
+    std::lock_guard<std::mutex> lk(ctx().mu);
+    for (auto& w : pk->work) {
+      if (w.done) (void)hipEventDestroy(w.done);
+      for (DevBuf* b : {&w.keys, &w.vals, &w.counts, &w.skeys, &w.svals, &w.buckets, &w.pkeys, &w.pflags, &w.ppts,
+                        &w.lvlS, &w.lvlT, &w.outPQ, &w.scan_tmp, &w.red_scratch})
+        b->release();
+      if (w.host_out.ptr) (void)hipHostFree(w.host_out.ptr);
+    }
+    if (pk->stage.ptr) (void)hipHostFree(pk->stage.ptr);
+    delete pk;
+  });
+}
+
+extern "C" int bg_pk_header(const bg_pk* pk, uint32_t* header8) {
+  return guard([&] {
+    BG_REQUIRE(pk && header8, BG_ERR_ARG, "bg_pk_header: null argument");
+    memcpy(header8, pk->hdr, 32);
+  });
+}
+
+extern "C" int bg_pk_device_bytes(const bg_pk* pk, uint64_t* bytes) {
+  return guard([&] {
+    BG_REQUIRE(pk && bytes, BG_ERR_ARG, "bg_pk_device_bytes: null argument");
+    *bytes = pk->dev_bytes;
+  });
+}
+
+extern "C" int bg_prove(bg_pk* pk, const uint32_t* witness, const uint32_t* rs, uint32_t* proof_out, int flags,
+                        double* t_ms) {
+  return guard([&] {
+    BG_REQUIRE(pk && witness && rs && proof_out, BG_ERR_ARG, "bg_prove: null argument");
+    bool words16 = (flags & 1) == BG_FMT_WORDS16;
+    bool dev_witness = (flags & BG_MEM_DEVICE) != 0;
+    BG_REQUIRE(!(dev_witness && words16), BG_ERR_ARG, "bg_prove: a device witness must be packed");
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto t0 = std::chrono::steady_clock::now();
+    const uint32_t nc = pk->hdr[0], ni = pk->hdr[1], nw = pk->hdr[2], lg = pk->hdr[3];
+    const uint32_t na = pk->hdr[4], nbq = pk->hdr[5], nh = pk->hdr[6], nl = pk->hdr[7];
+    const uint64_t d = 1ull << lg, nz = (uint64_t)ni + nw;
+    hipStream_t s = c.stream;
+    hipEvent_t ev[9];
+    for (auto& e : ev) BG_HIP(hipEventCreate(&e));
+    BG_HIP(hipEventRecord(ev[0], s));
+    // 1. witness upload (16-bit words are packed on the device)
+    uint32_t* z = pk->z.as<uint32_t>();
+    if (dev_witness) {
+      BG_HIP(hipMemcpyAsync(z, witness, 32 * nz, hipMemcpyDeviceToDevice, s));
+    } else if (words16) {
+      BG_HIP(hipMemcpyAsync(pk->zraw.ptr, witness, 64 * nz, hipMemcpyHostToDevice, s));
+      launch_pack16(pk->zraw.as<uint32_t>(), z, 8 * nz, s);
+    } else {
+      BG_HIP(hipMemcpyAsync(z, witness, 32 * nz, hipMemcpyHostToDevice, s));
+    }
+    BG_HIP(hipEventRecord(ev[1], s));
+    // 2. evaluation vectors and canonical scalars
+    uint32_t* zc = pk->zc.as<uint32_t>();
+    {
+      ScopedKTime kt("evals", s);
+      k_evals<<<nb(d, 256), 256, 0, s>>>(pk->A.offs.as<uint32_t>(), pk->A.cols.as<uint32_t>(), pk->A.coef.as<uint32_t>(),
+                                         z, nc, ni, 1, d, pk->ea.as<uint32_t>());
+      k_evals<<<nb(d, 256), 256, 0, s>>>(pk->B.offs.as<uint32_t>(), pk->B.cols.as<uint32_t>(), pk->B.coef.as<uint32_t>(),
+                                         z, nc, ni, 0, d, pk->eb.as<uint32_t>());
+      k_evals<<<nb(d, 256), 256, 0, s>>>(pk->C.offs.as<uint32_t>(), pk->C.cols.as<uint32_t>(), pk->C.coef.as<uint32_t>(),
+                                         z, nc, ni, 0, d, pk->ec.as<uint32_t>());
+      k_from_mont<<<nb(nz, 256), 256, 0, s>>>(z, zc, nz);
+      BG_HIP(hipGetLastError());
+    }
+    BG_HIP(hipEventRecord(ev[2], s));
+    // 3. h (canonical, the MSM H scalars)
+    {
+      ScopedKTime kt("qap_h", s);
+      qap_h_run(pk->ea.as<uint32_t>(), pk->eb.as<uint32_t>(), pk->ec.as<uint32_t>(), pk->h.as<uint32_t>(),
+                pk->tmp.as<uint32_t>(), (int)lg, true, s);
+    }
+    BG_HIP(hipEventRecord(ev[3], s));
+    // 4. MSMs in the reference order: A, B1, B2, H, L (prove.bend:64-116)
+    msm_enqueue(1, pk->pa.as<uint32_t>(), zc, na, 0, s, pk->work[0]);
+    BG_HIP(hipEventRecord(ev[4], s));
+    msm_enqueue(1, pk->pb1.as<uint32_t>(), zc, nbq, 0, s, pk->work[1]);
+    BG_HIP(hipEventRecord(ev[5], s));
+    msm_enqueue(2, pk->pb2.as<uint32_t>(), zc, nbq, 0, s, pk->work[2]);
+    BG_HIP(hipEventRecord(ev[6], s));
+    msm_enqueue(1, pk->ph.as<uint32_t>(), pk->h.as<uint32_t>(), nh, 0, s, pk->work[3]);
+    BG_HIP(hipEventRecord(ev[7], s));
+    msm_enqueue(1, pk->pl.as<uint32_t>(), zc + 8 * ni, nl, 0, s, pk->work[4]);
+    BG_HIP(hipEventRecord(ev[8], s));
+
+    // host work that depends only on r, s overlaps with the device
+    uint64_t rc[4], sc[4], rsc[4];
+    canon_words(rs, rc);
+    canon_words(rs + 8, sc);
+    {
+      bgh::Fr rs_m = bgh::Fr::from_u32(rs) * bgh::Fr::from_u32(rs + 8);
+      bgh::Fr x = rs_m.from_mont();
+      memcpy(rsc, x.v, 32);
+    }
+    bgh::P1 alpha = bgh::load_affine<bgh::Fq>(pk->g1), beta1 = bgh::load_affine<bgh::Fq>(pk->g1 + 16),
+       delta1 = bgh::load_affine<bgh::Fq>(pk->g1 + 32);
+    bgh::P2 beta2 = bgh::load_affine<bgh::Fq2>(pk->g2), delta2 = bgh::load_affine<bgh::Fq2>(pk->g2 + 32);
+    bgh::P1 r_d1 = delta1.mul(rc), s_d1 = delta1.mul(sc), rs_d1 = delta1.mul(rsc);
+    bgh::P2 s_d2 = delta2.mul(sc);
+
+    // 5. finish the MSMs and combine (groth16.bend:298-308)
+    uint32_t mA[16], mB1[16], mB2[32], mH[16], mL[16];
+    msm_finish(pk->work[0], mA);
+    msm_finish(pk->work[1], mB1);
+    bgh::P1 A = bgh::load_affine<bgh::Fq>(mA).add(r_d1).add(alpha);
+    bgh::P1 B1 = bgh::load_affine<bgh::Fq>(mB1).add(s_d1).add(beta1);
+    bgh::P1 sA_rB = A.mul(sc).add(B1.mul(rc));
+    msm_finish(pk->work[2], mB2);
+    bgh::P2 B2 = bgh::load_affine<bgh::Fq2>(mB2).add(s_d2).add(beta2);
+    msm_finish(pk->work[3], mH);
+    msm_finish(pk->work[4], mL);
+    bgh::P1 C = sA_rB.add(rs_d1.neg()).add(bgh::load_affine<bgh::Fq>(mL)).add(bgh::load_affine<bgh::Fq>(mH));
+    uint32_t proof[64];
+    bgh::store_affine(A, proof);
+    bgh::store_affine(B2, proof + 16);
+    bgh::store_affine(C, proof + 48);
+    if (words16) {
+      for (int i = 0; i < 64; i++) {
+        proof_out[2 * i] = proof[i] & 0xFFFFu;
+        proof_out[2 * i + 1] = proof[i] >> 16;
+      }
+    } else {
+      memcpy(proof_out, proof, sizeof proof);
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    if (t_ms) {
+      // device time of each phase (one stream, so the phases run back to back), then
+      // the wall time of the whole call
+      for (int i = 0; i < 8; i++) {
+        float e = 0;
+        BG_HIP(hipEventElapsedTime(&e, ev[i], ev[i + 1]));
+        t_ms[i] = e;
+      }
+      t_ms[8] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+  });
+}
+
+static void hex_be(const uint32_t* packed_mont, char* out) {
+  // canonical big-endian "0x" + 64 hex digits (bend2/src/fq.bend:2482-2489)
+  bgh::Fq x = bgh::Fq::from_u32(packed_mont).from_mont();
+  static const char* H = "0123456789abcdef";
+  out[0] = '0';
+  out[1] = 'x';
+  for (int i = 0; i < 64; i++) {
+    int nib = 63 - i;
+    uint64_t w = x.v[nib / 16];
+    out[2 + i] = H[(w >> (4 * (nib % 16))) & 0xF];
+  }
+  out[66] = 0;
+}
+
+extern "C" int bg_proof_format(const uint32_t* p, char* buf, int cap) {
+  int n = 0;
+  int rc = guard([&] {
+    BG_REQUIRE(p && buf && cap > 0, BG_ERR_ARG, "bg_proof_format: bad argument");
+    char h[8][67];
+    const int off[8] = {0, 8, 16, 24, 32, 40, 48, 56};
+    for (int i = 0; i < 8; i++) hex_be(p + off[i], h[i]);
+    n = snprintf(buf, (size_t)cap, "A %s %s\nB %s %s %s %s\nC %s %s\n", h[0], h[1], h[2], h[3], h[4], h[5], h[6],
+                 h[7]);
+    BG_REQUIRE(n < cap, BG_ERR_ARG, "bg_proof_format: buffer of %d bytes too small", cap);
+  });
+  return rc == BG_OK ? n : rc;
+}

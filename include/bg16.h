@@ -67,11 +67,17 @@ int bg_timing_reset(void);
  * order r).  out: one affine point (same encoding as points; infinity = zeros).  The
  * reference returns Jacobian; the affine point is unique, and the reference's caller
  * normalises it (bend2/src/groth16.bend:281-293).
- * window_bits: 0 = tuned default, else the signed-digit window width c (4..20). */
+ * window_bits: 0 = tuned default, else the signed-digit window width c (2..22). */
 int bg_msm_g1(const uint32_t* points, const uint32_t* scalars, uint64_t n, uint32_t* out,
               int flags, int window_bits);
 int bg_msm_g2(const uint32_t* points, const uint32_t* scalars, uint64_t n, uint32_t* out,
               int flags, int window_bits);
+/* The MSM schedule chosen for n points of group (1 G1, 2 G2): info8 = window bits c,
+ * windows W, buckets per window, sorted entries (W n), entries per accumulation thread,
+ * accumulation threads, level-1 groups, level-2 outputs.  No counterpart in the
+ * reference (its window choice is Msm.pick_c, bend2/src/msm.bend:40-44); used to state
+ * the algorithmic work of a launch. */
+int bg_msm_plan_info(uint64_t n, int group, int window_bits, uint64_t* info8);
 
 /* ------------------------------------------------------------------ NTT
  * Replaces Ntt.fft / ifft / coset_fft / coset_ifft (bend2/src/ntt.bend:431-442), in
@@ -100,10 +106,12 @@ int bg_pk_header(const bg_pk* pk, uint32_t* header8);
 int bg_pk_device_bytes(const bg_pk* pk, uint64_t* bytes);
 
 /* One proof.  witness: num_instance + num_witness Fr (Montgomery, FORMAT.md witness.bin
- * order); rs: r then s (Montgomery, FORMAT.md rs.bin).  proof_out: A (G1), B (G2),
- * C (G1) affine Montgomery (the proof_ref.bin layout), in the flags' encoding (host).
- * t_ms (optional, 8 doubles): upload, evals, qap, msm_a, msm_b1, msm_b2, msm_h+msm_l,
- * finish -- phase wall times. */
+ * order), host memory in either encoding, or with BG_MEM_DEVICE a packed device buffer
+ * already resident in HBM; rs: r then s (Montgomery, FORMAT.md rs.bin, host).
+ * proof_out (host): A (G1), B (G2), C (G1) affine Montgomery (the proof_ref.bin layout),
+ * in the flags' encoding.
+ * t_ms (optional, 9 doubles): device time of upload, evals, qap, msm_a, msm_b1, msm_b2,
+ * msm_h, msm_l (the T phases of bend2/prove.bend:154-162), then the call's wall time. */
 int bg_prove(bg_pk* pk, const uint32_t* witness, const uint32_t* rs, uint32_t* proof_out,
              int flags, double* t_ms);
 

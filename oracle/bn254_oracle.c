@@ -966,3 +966,14 @@ void or_fr_dot(u64 *out, const u64 *a, const u64 *k, u64 n) {
     }
     memcpy(out, acc, 32);
 }
+
+/* Horner evaluation of a Montgomery-coefficient polynomial at Montgomery x */
+void or_poly_eval(u64 *out, const u64 *coeffs, u64 n, const u64 *x) {
+    or_init_once();
+    u64 acc[4] = {0, 0, 0, 0};
+    for (u64 i = n; i-- > 0;) {
+        f_mul(&FR, acc, acc, x);
+        f_add(&FR, acc, acc, coeffs + 4 * i);
+    }
+    memcpy(out, acc, 32);
+}

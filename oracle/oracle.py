@@ -82,8 +82,14 @@ def mont(x: int, p: int) -> int:
     return x * R256 % p
 
 
+_RINV = {}
+
+
 def unmont(x: int, p: int) -> int:
-    return x * pow(R256, -1, p) % p
+    ri = _RINV.get(p)
+    if ri is None:
+        ri = _RINV[p] = pow(R256, -1, p)
+    return x * ri % p
 
 
 def ints_to_array(xs, mont_p: int | None = None) -> np.ndarray:
@@ -310,7 +316,7 @@ def lib():
             "or_prove": [P] * 23 + [P],
             "or_threads": [], "or_set_threads": [ctypes.c_int],
             "or_g1_gen_mul_batch": [P, P, U64], "or_g2_gen_mul_batch": [P, P, U64],
-            "or_fr_dot": [P, P, P, U64],
+            "or_fr_dot": [P, P, P, U64], "or_poly_eval": [P, P, U64, P],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -462,6 +468,15 @@ def rand_scalars(rng: np.random.Generator, n: int, bound: int = P_R) -> np.ndarr
                 out[i] = rng.integers(0, 2**64, size=4, dtype=np.uint64)
                 out[i, 3] = rng.integers(0, top + 1, dtype=np.uint64)
     return out
+
+
+def poly_eval(coeffs: np.ndarray, x_mont: int) -> int:
+    """Horner evaluation (Montgomery in/out, (n, 4) u64 coefficients)."""
+    c = np.ascontiguousarray(coeffs, dtype=np.uint64)
+    x = np.array(to_words(x_mont), dtype=np.uint64)
+    out = np.zeros(4, dtype=np.uint64)
+    lib().or_poly_eval(_ptr(out), _ptr(c), c.shape[0], _ptr(x))
+    return from_words(out)
 
 
 def set_threads(n: int) -> None:

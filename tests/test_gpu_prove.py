@@ -1,0 +1,101 @@
+"""End-to-end GPU prove (bend2/prove.bend's Pr.run on the device) against the CPU oracle
+prover and the trapdoor closed form, bit-exact; repeated proofs are identical."""
+import numpy as np
+import pytest
+
+import bendg
+from oracle import oracle as O
+from instance import closed_form, load, points_of
+
+pytestmark = pytest.mark.gpu
+
+
+def _files(d):
+    z = np.fromfile(d / "witness.bin", dtype=np.uint32)
+    rs = np.fromfile(d / "rs.bin", dtype=np.uint32)
+    return z, rs
+
+
+def _split(proof_packed):
+    p = proof_packed.view(np.uint64)
+    return p[:8], p[8:24], p[24:32]
+
+
+@pytest.fixture(scope="module")
+def keys(tmp_path_factory):
+    out = {}
+    for lg in (3, 4, 10, 12):
+        d = tmp_path_factory.mktemp(f"k{lg}")
+        bendg.setup_square_chain(d, lg, seed=42 + lg)
+        out[lg] = d
+    return out
+
+
+@pytest.mark.parametrize("lg", [3, 4, 10, 12])
+def test_prove_vs_oracle_and_closed_form(bg, keys, lg):
+    d = keys[lg]
+    print(f"prove K={lg}", flush=True)
+    pk = bendg.ProvingKey(d)
+    z, rs = _files(d)
+    proof16, t = pk.prove(z, rs, fmt="words16")
+    A, B, C = _split(bendg.pack16(proof16.reshape(-1, 16)).reshape(-1))
+    inst = load(d)
+    oA, oB, oC, _ = O.prove(inst)
+    assert np.array_equal(A, oA) and np.array_equal(B, oB) and np.array_equal(C, oC)
+    eA, eB, eC = points_of(closed_form(inst))
+    assert np.array_equal(A, eA) and np.array_equal(B, eB) and np.array_equal(C, eC)
+    pk.close()
+
+
+def test_prove_packed_and_repeat(bg, keys):
+    d = keys[10]
+    pk = bendg.ProvingKey(d)
+    z, rs = _files(d)
+    zp = bendg.pack16(z.reshape(-1, 16)).reshape(-1)
+    rp = bendg.pack16(rs.reshape(-1, 16)).reshape(-1)
+    first, _ = pk.prove(zp, rp, fmt="packed")
+    for _ in range(3):
+        again, _ = pk.prove(zp, rp, fmt="packed")
+        assert np.array_equal(first, again)
+    p16, _ = pk.prove(z, rs, fmt="words16")
+    assert np.array_equal(bendg.pack16(p16.reshape(-1, 16)).reshape(-1), first)
+    text = bendg.format_proof(first)
+    lines = text.strip().split("\n")
+    assert [l.split()[0] for l in lines] == ["A", "B", "C"] and len(lines[1].split()) == 5
+    ax = int(lines[0].split()[1], 16)
+    assert ax == O.unmont(O.from_words(first.view(np.uint64)[:4]), O.P_Q)
+    pk.close()
+
+
+def test_two_keys_resident(bg, keys):
+    """two proving keys resident at once give the same proofs as alone"""
+    pa, pb = bendg.ProvingKey(keys[4]), bendg.ProvingKey(keys[10])
+    za, ra = _files(keys[4])
+    zb, rb = _files(keys[10])
+    a1, _ = pa.prove(za, ra)
+    b1, _ = pb.prove(zb, rb)
+    a2, _ = pa.prove(za, ra)
+    assert np.array_equal(a1, a2)
+    inst = load(keys[10])
+    oA, oB, oC, _ = O.prove(inst)
+    A, B, C = _split(bendg.pack16(b1.reshape(-1, 16)).reshape(-1))
+    assert np.array_equal(A, oA) and np.array_equal(C, oC)
+    pa.close()
+    pb.close()
+
+
+@pytest.mark.parametrize("lg", [16, 18])
+def test_prove_large_closed_form(bg, tmp_path, lg):
+    print(f"setup K={lg}", flush=True)
+    bendg.setup_square_chain(tmp_path, lg, seed=42)
+    print(f"prove K={lg}", flush=True)
+    pk = bendg.ProvingKey(tmp_path)
+    z, rs = _files(tmp_path)
+    proof16, t = pk.prove(z, rs, fmt="words16")
+    proof16, t = pk.prove(z, rs, fmt="words16")
+    print(f"K={lg} phases: " + " ".join(f"{k}={v:.2f}" for k, v in t.items()), flush=True)
+    A, B, C = _split(bendg.pack16(proof16.reshape(-1, 16)).reshape(-1))
+    print("closed form", flush=True)
+    eA, eB, eC = points_of(closed_form(load(tmp_path)))
+    assert np.array_equal(A, eA) and np.array_equal(B, eB) and np.array_equal(C, eC)
+    pk.close()
