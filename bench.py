@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark: one Groth16 proof of SquareChain(2^18) on one MI355X per step.
 
-Workload (the reference's headline configuration, configs[1] of the benchmark plan):
+Workload (the reference's headline configuration, the single 2^18 proof):
 SquareChain with N = 2^K constraints (x_{i+1} = x_i^2), domain d = 2^(K+1), proving key
 from this library's seeded trapdoor setup (synthetic, seed 42), key resident in HBM and
 the packed witness already resident in HBM when the timed region starts.  A step is a

@@ -311,11 +311,16 @@ extern "C" int bg_prove(bg_pk* pk, const uint32_t* witness, const uint32_t* rs, 
     BG_HIP(hipEventRecord(ev[8], s));
 
     // host work that depends only on r, s overlaps with the device
+    uint32_t rs_packed[16];
+    if (words16)
+      pack_words(rs, rs_packed, 16, "rs");
+    else
+      memcpy(rs_packed, rs, sizeof rs_packed);
     uint64_t rc[4], sc[4], rsc[4];
-    canon_words(rs, rc);
-    canon_words(rs + 8, sc);
+    canon_words(rs_packed, rc);
+    canon_words(rs_packed + 8, sc);
     {
-      bgh::Fr rs_m = bgh::Fr::from_u32(rs) * bgh::Fr::from_u32(rs + 8);
+      bgh::Fr rs_m = bgh::Fr::from_u32(rs_packed) * bgh::Fr::from_u32(rs_packed + 8);
       bgh::Fr x = rs_m.from_mont();
       memcpy(rsc, x.v, 32);
     }
