@@ -74,6 +74,7 @@ def lib() -> ctypes.CDLL:
         "bg_curve_op": [I, I, P, P, P, U64],
         "bg_probe_rate": [I, U64, P],
         "bg_msm_plan_info": [U64, I, I, P],
+        "bg_msm_batch": [I, ctypes.c_uint32, P, P, P, P, I, I],
     }
     for name, args in sigs.items():
         fn = getattr(L, name, None)
@@ -204,6 +205,25 @@ def msm_g2(points: np.ndarray, scalars: np.ndarray, fmt: str = "packed", window_
         raise ValueError("fewer points than scalars")
     out = np.zeros(4 * ew, dtype=np.uint32)
     _check(lib().bg_msm_g2(_p(points), _p(scalars), n, _p(out), _fmt_flag(fmt), window_bits))
+    return out
+
+
+def msm_batch(group: int, points: list, scalars: list, window_bits: int = 0) -> np.ndarray:
+    """Batch of independent MSMs of one group (packed host arrays) in one launch sequence;
+    returns (count, 16 | 32) packed affine results."""
+    count = len(points)
+    pw = 16 if group == 1 else 32
+    ps = [_u32(p).reshape(-1, pw) for p in points]
+    ss = [_u32(s).reshape(-1, 8) for s in scalars]
+    ns = np.array([s.shape[0] for s in ss], dtype=np.uint64)
+    for p, s in zip(ps, ss):
+        if p.shape[0] < s.shape[0]:
+            raise ValueError("fewer points than scalars")
+    pp = (ctypes.c_void_p * count)(*[p.ctypes.data for p in ps])
+    sp = (ctypes.c_void_p * count)(*[s.ctypes.data for s in ss])
+    out = np.zeros((count, pw), dtype=np.uint32)
+    _check(lib().bg_msm_batch(group, count, ctypes.cast(pp, ctypes.c_void_p), ctypes.cast(sp, ctypes.c_void_p),
+                              _p(ns), _p(out), FMT_PACKED, window_bits))
     return out
 
 

@@ -86,6 +86,54 @@ struct Curve {
     return {x3, y3, F::mul(p.ZZ, pp), F::mul(p.ZZZ, ppp)};
   }
 
+  // p + a by the generic madd-2008-s formula only, for p not infinity and a not
+  // infinity; sets bad when x(p) = x(a) (doubling or cancellation), where the result is
+  // not the sum and the caller must recompute exactly.
+  __device__ __forceinline__ static P madd_generic(const P& p, const A& a, uint32_t& bad) {
+    E u2 = F::mul(a.x, p.ZZ);
+    E pp_ = F::sub(u2, p.X);
+    bad |= F::is_zero(pp_) ? 1u : 0u;
+    E s2 = F::mul(a.y, p.ZZZ);
+    E r = F::sub(s2, p.Y);
+    E pp = F::sqr(pp_);
+    E zz3 = F::mul(p.ZZ, pp);
+    E ppp = F::mul(pp_, pp);
+    E zzz3 = F::mul(p.ZZZ, ppp);
+    E q = F::mul(p.X, pp);
+    E x3 = F::sub(F::sub(F::sqr(r), ppp), F::dbl(q));
+    E y3 = F::sub(F::mul(r, F::sub(q, x3)), F::mul(p.Y, ppp));
+    return {x3, y3, zz3, zzz3};
+  }
+
+  // p + q by the generic add-2008-s formula, for p and q not infinity; sets bad when
+  // x(p) = x(q), where the result is not the sum.
+  __device__ __forceinline__ static P add_generic(const P& p, const P& q, uint32_t& bad) {
+    E u1 = F::mul(p.X, q.ZZ);
+    E u2 = F::mul(q.X, p.ZZ);
+    E pp_ = F::sub(u2, u1);
+    bad |= F::is_zero(pp_) ? 1u : 0u;
+    E s1 = F::mul(p.Y, q.ZZZ);
+    E s2 = F::mul(q.Y, p.ZZZ);
+    E r = F::sub(s2, s1);
+    E pp = F::sqr(pp_);
+    E ppp = F::mul(pp_, pp);
+    E q_ = F::mul(u1, pp);
+    E zz3 = F::mul(F::mul(p.ZZ, q.ZZ), pp);
+    E zzz3 = F::mul(F::mul(p.ZZZ, q.ZZZ), ppp);
+    E x3 = F::sub(F::sub(F::sqr(r), ppp), F::dbl(q_));
+    E y3 = F::sub(F::mul(r, F::sub(q_, x3)), F::mul(s1, ppp));
+    return {x3, y3, zz3, zzz3};
+  }
+
+  // p + q where either may be infinity (ZZ = 0): the generic formula, with the infinite
+  // operand cases selected around it; bad as in add_generic.
+  __device__ __forceinline__ static P add_lite(const P& p, const P& q, uint32_t& bad) {
+    bool pi = is_inf(p), qi = is_inf(q);
+    if (pi) return q;
+    if (qi) return p;
+    return add_generic(p, q, bad);
+  }
+
   // p + q: add-2008-s with exact special cases
   __device__ __forceinline__ static P add(const P& p, const P& q) {
     if (is_inf(p)) return q;

@@ -247,16 +247,25 @@ struct Fq2 {
   __device__ __forceinline__ static Fp2 sub(const Fp2& a, const Fp2& b) { return {Fq::sub(a.c0, b.c0), Fq::sub(a.c1, b.c1)}; }
   __device__ __forceinline__ static Fp2 neg(const Fp2& a) { return {Fq::neg(a.c0), Fq::neg(a.c1)}; }
   __device__ __forceinline__ static Fp2 dbl(const Fp2& a) { return {Fq::dbl(a.c0), Fq::dbl(a.c1)}; }
-  // Karatsuba: 3 Fq products
+  // Karatsuba: 3 Fq products.  The scheduling barriers keep the three products from
+  // being interleaved, which would hold three accumulators and operand sets live at once
+  // (the G2 accumulation then no longer fits two waves per SIMD).
   __device__ __forceinline__ static Fp2 mul(const Fp2& a, const Fp2& b) {
     Fp t0 = Fq::mul(a.c0, b.c0);
+    __builtin_amdgcn_sched_barrier(0);
     Fp t1 = Fq::mul(a.c1, b.c1);
+    __builtin_amdgcn_sched_barrier(0);
     Fp t2 = Fq::mul(Fq::add(a.c0, a.c1), Fq::add(b.c0, b.c1));
+    __builtin_amdgcn_sched_barrier(0);
     return {Fq::sub(t0, t1), Fq::sub(Fq::sub(t2, t0), t1)};
   }
   // complex squaring: 2 Fq products
   __device__ __forceinline__ static Fp2 sqr(const Fp2& a) {
-    return {Fq::mul(Fq::add(a.c0, a.c1), Fq::sub(a.c0, a.c1)), Fq::mul(Fq::dbl(a.c0), a.c1)};
+    Fp c0 = Fq::mul(Fq::add(a.c0, a.c1), Fq::sub(a.c0, a.c1));
+    __builtin_amdgcn_sched_barrier(0);
+    Fp c1 = Fq::mul(Fq::dbl(a.c0), a.c1);
+    __builtin_amdgcn_sched_barrier(0);
+    return {c0, c1};
   }
   __device__ __noinline__ static Fp2 inv(const Fp2& a) {
     Fp n = Fq::add(Fq::sqr(a.c0), Fq::sqr(a.c1));

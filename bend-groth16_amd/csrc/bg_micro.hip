@@ -171,6 +171,61 @@ __global__ void k_probe_madd(uint64_t iters, uint32_t seed, uint32_t* sink) {
   if ((acc.X.v[0] ^ acc.ZZZ.v[1]) == 0x1234567u) sink[0] = 1;
 }
 
+// latency probes: ILP independent dependency chains per lane
+template <int ILP>
+__global__ void k_probe_mad_chain(uint64_t iters, uint32_t seed, uint64_t* sink) {
+  uint32_t t = threadIdx.x + blockIdx.x * blockDim.x;
+  uint64_t acc[ILP];
+#pragma unroll
+  for (int k = 0; k < ILP; k++) acc[k] = ((uint64_t)(seed * (k + 1)) << 32) | (t + k);
+  uint32_t y = seed ^ t;
+  for (uint64_t it = 0; it < iters; it++) {
+#pragma unroll
+    for (int r = 0; r < 16 / ILP; r++)
+#pragma unroll
+      for (int k = 0; k < ILP; k++) acc[k] = (uint64_t)(uint32_t)acc[k] * y + acc[k];
+  }
+  uint64_t s = 0;
+#pragma unroll
+  for (int k = 0; k < ILP; k++) s ^= acc[k];
+  if (s == 0x1234567ull) sink[0] = s;
+}
+
+template <int ILP>
+__global__ void k_probe_fqmul_ilp(uint64_t iters, uint32_t seed, uint32_t* sink) {
+  uint32_t t = threadIdx.x + blockIdx.x * blockDim.x;
+  Fp a[ILP], b;
+#pragma unroll
+  for (int k = 0; k < ILP; k++)
+#pragma unroll
+    for (int j = 0; j < 8; j++) a[k].v[j] = (seed * (j + 7 * k + 1) + t) & (j == 7 ? 0x0fffffffu : 0xffffffffu);
+#pragma unroll
+  for (int j = 0; j < 8; j++) b.v[j] = (seed ^ (j * 977 + t)) & (j == 7 ? 0x0fffffffu : 0xffffffffu);
+  for (uint64_t it = 0; it < iters; it++) {
+#pragma unroll
+    for (int k = 0; k < ILP; k++) a[k] = Fq::mul(a[k], b);
+  }
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < ILP; k++) s ^= a[k].v[0];
+  if (s == 0x1234567u) sink[0] = s;
+}
+
+__global__ void k_probe_fq2mul(uint64_t iters, uint32_t seed, uint32_t* sink) {
+  uint32_t t = threadIdx.x + blockIdx.x * blockDim.x;
+  Fp2 a, b;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    uint32_t m = j == 7 ? 0x0fffffffu : 0xffffffffu;
+    a.c0.v[j] = (seed * (j + 1) + t) & m;
+    a.c1.v[j] = (seed * (j + 3) ^ t) & m;
+    b.c0.v[j] = (seed * (j + 5) + 3 * t) & m;
+    b.c1.v[j] = (seed * (j + 9) ^ 5 * t) & m;
+  }
+  for (uint64_t it = 0; it < iters; it++) a = Fq2::mul(a, b);
+  if ((a.c0.v[0] ^ a.c1.v[1]) == 0x1234567u) sink[0] = 1;
+}
+
 }  // namespace bg
 
 using namespace bg;
@@ -249,23 +304,39 @@ extern "C" int bg_curve_op(int group, int op, const uint32_t* p, const uint32_t*
   });
 }
 
+// Throughput / latency probes.  kind: 0 v_mad_u64_u32 (16 independent chains, 16 waves
+// per CU), 1 Fq products (4 independent per lane), 2 G1 mixed adds, 3/4 v_mad chains with
+// ILP 1/4 at one wave per SIMD, 5/6 Fq products ILP 1 at one / four waves per SIMD,
+// 7 Fq products ILP 4 at one wave per SIMD, 8 Fq2 products at one wave per SIMD,
+// 9 Fq2 products at two waves per SIMD.  result[0] = operations/s, result[1] = ms.
 extern "C" int bg_probe_rate(int kind, uint64_t iters, double* result) {
   return guard([&] {
-    BG_REQUIRE(kind >= 0 && kind <= 2 && result && iters > 0, BG_ERR_ARG, "bg_probe_rate: bad argument");
+    BG_REQUIRE(kind >= 0 && kind <= 9 && result && iters > 0, BG_ERR_ARG, "bg_probe_rate: bad argument");
     Ctx& c = ctx();
     uint32_t* sink = c.scratch[0].as<uint32_t>(4);
-    int threads = 256;
-    int blocks = c.cu_count * 16;
+    uint64_t* sink64 = reinterpret_cast<uint64_t*>(sink);
+    const int threads = 256;  // 4 waves: one per SIMD of a CU
+    int wps = 4;              // waves per SIMD
+    if (kind == 3 || kind == 4 || kind == 5 || kind == 7 || kind == 8) wps = 1;
+    if (kind == 9) wps = 2;
+    if (kind == 6) wps = 4;
+    int blocks = c.cu_count * wps;
+    double per_thread = 1.0;
     hipEvent_t e0, e1;
     BG_HIP(hipEventCreate(&e0));
     BG_HIP(hipEventCreate(&e1));
     auto launch = [&](uint64_t it) {
-      if (kind == 0)
-        k_probe_mad<<<blocks, threads, 0, c.stream>>>(it, 12345u, reinterpret_cast<uint64_t*>(sink));
-      else if (kind == 1)
-        k_probe_fqmul<<<blocks, threads, 0, c.stream>>>(it, 12345u, sink);
-      else
-        k_probe_madd<<<blocks, threads, 0, c.stream>>>(it, 12345u, sink);
+      switch (kind) {
+        case 0: k_probe_mad<<<blocks, threads, 0, c.stream>>>(it, 12345u, sink64); per_thread = 16; break;
+        case 1: k_probe_fqmul<<<blocks, threads, 0, c.stream>>>(it, 12345u, sink); per_thread = 4; break;
+        case 2: k_probe_madd<<<blocks, threads, 0, c.stream>>>(it, 12345u, sink); per_thread = 1; break;
+        case 3: k_probe_mad_chain<1><<<blocks, threads, 0, c.stream>>>(it, 12345u, sink64); per_thread = 16; break;
+        case 4: k_probe_mad_chain<4><<<blocks, threads, 0, c.stream>>>(it, 12345u, sink64); per_thread = 16; break;
+        case 5:
+        case 6: k_probe_fqmul_ilp<1><<<blocks, threads, 0, c.stream>>>(it, 12345u, sink); per_thread = 1; break;
+        case 7: k_probe_fqmul_ilp<4><<<blocks, threads, 0, c.stream>>>(it, 12345u, sink); per_thread = 4; break;
+        default: k_probe_fq2mul<<<blocks, threads, 0, c.stream>>>(it, 12345u, sink); per_thread = 1; break;
+      }
     };
     launch(1);  // warm-up
     BG_HIP(hipEventRecord(e0, c.stream));
@@ -274,7 +345,6 @@ extern "C" int bg_probe_rate(int kind, uint64_t iters, double* result) {
     BG_HIP(hipEventSynchronize(e1));
     float ms = 0;
     BG_HIP(hipEventElapsedTime(&ms, e0, e1));
-    double per_thread = kind == 0 ? 16.0 : (kind == 1 ? 4.0 : 1.0);
     result[0] = (double)blocks * threads * per_thread * (double)iters / (ms * 1e-3);
     result[1] = ms;
     (void)hipEventDestroy(e0);

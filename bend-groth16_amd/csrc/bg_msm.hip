@@ -12,38 +12,55 @@ static constexpr uint32_t NOKEY = 0xFFFFFFFFu;
 
 // ------------------------------------------------------------------ planning
 
-MsmPlan msm_plan(uint64_t n, int group, int window_bits) {
+// Tuning overrides (BG16_MSM_C_G1 / BG16_MSM_C_G2 / BG16_MSM_L / BG16_MSM_M / BG16_MSM_K);
+// 0 or unset keeps the planner's choice.
+static int tune_env(const char* name) {
+  const char* e = getenv(name);
+  return e ? atoi(e) : 0;
+}
+
+MsmPlan msm_plan_batch(const uint64_t* ns, uint32_t M, int group, int window_bits) {
   MsmPlan p;
+  if (window_bits <= 0) window_bits = tune_env(group == 1 ? "BG16_MSM_C_G1" : "BG16_MSM_C_G2");
+  p.M = M;
+  uint64_t n = 0;
+  for (uint32_t i = 0; i < M; i++) n += ns[i];
   p.n = n;
   int c = window_bits;
   if (c <= 0) {
-    // minimise n*W*madd + W*2^(c-1)*2*add in field products (G1 10/14, G2 30/42: same ratio)
+    // minimise n*W*madd + M*W*2^(c-1)*2*add in field products (G1 10/14; G2 has the same ratio)
     double best = 1e300;
     for (int cc = 4; cc <= 20; cc++) {
       int W = (255 + cc - 1) / cc;
-      double cost = (double)n * W * 10.0 + (double)W * (double)(1u << (cc - 1)) * 28.0;
+      double cost = (double)n * W * 10.0 + (double)M * W * (double)(1u << (cc - 1)) * 28.0;
       if (cost < best) {
         best = cost;
         c = cc;
       }
     }
   }
-  (void)group;
   p.c = c;
   p.W = (255 + c - 1) / c;
   p.B = 1u << (c - 1);
-  p.nb = (uint32_t)p.W * p.B;
-  p.m = std::min<uint32_t>(8, p.B);
+  p.kspan = (uint32_t)p.W * p.B;
+  p.nb = M * p.kspan;
+  uint32_t mm = (uint32_t)tune_env("BG16_MSM_M");
+  p.m = std::min<uint32_t>(mm ? mm : 8, p.B);
   p.G = p.B / p.m;
   uint32_t kmax = group == 1 ? 256 : 128;
+  uint32_t kk = (uint32_t)tune_env("BG16_MSM_K");
+  if (kk) kmax = std::min(kmax, kk);
   p.K = std::min<uint32_t>(kmax, p.G);
   p.H = p.G / p.K;
   p.entries = (uint64_t)p.W * n;
   uint64_t L = p.entries / 16384;
-  p.L = (uint32_t)std::max<uint64_t>(2, std::min<uint64_t>(16, L));
+  uint32_t lmax = (uint32_t)tune_env("BG16_MSM_L");
+  p.L = (uint32_t)std::max<uint64_t>(2, std::min<uint64_t>(lmax ? lmax : 16, L));
   p.acc_threads = (uint32_t)((p.entries + p.L - 1) / p.L);
   return p;
 }
+
+MsmPlan msm_plan(uint64_t n, int group, int window_bits) { return msm_plan_batch(&n, 1, group, window_bits); }
 
 // ------------------------------------------------------------------ kernels
 
@@ -71,8 +88,14 @@ __device__ __forceinline__ uint32_t get_bits(const uint32_t* s, int lo, int c) {
   return v & ((1u << c) - 1u);
 }
 
-__global__ void k_digits(const uint32_t* __restrict__ scalars, uint32_t n, int c, int W, uint32_t* __restrict__ keys,
-                         uint32_t* __restrict__ vals, uint32_t* __restrict__ counts) {
+// Signed c-bit digits of every scalar; for each nonzero digit the (bucket key, point
+// index | sign) pair and its rank inside the bucket (the value the histogram atomic
+// returns), so the scatter needs no atomics.
+// A base at infinity (all-zero affine encoding) contributes nothing and gets no entries,
+// so the accumulation never meets an infinite addend.
+__global__ void k_digits(const uint32_t* __restrict__ scalars, const uint32_t* __restrict__ points, int aff_words,
+                         uint32_t n, int c, int W, uint32_t key_base, uint32_t* __restrict__ keys,
+                         uint32_t* __restrict__ vals, uint32_t* __restrict__ ranks, uint32_t* __restrict__ counts) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint32_t s[8];
@@ -81,6 +104,16 @@ __global__ void k_digits(const uint32_t* __restrict__ scalars, uint32_t n, int c
   s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w;
   s[4] = b.x; s[5] = b.y; s[6] = b.z; s[7] = b.w;
   reduce_mod_r(s);
+  {
+    const uint4* pp = reinterpret_cast<const uint4*>(points + (uint64_t)aff_words * i);
+    uint32_t nz = 0;
+    for (int k = 0; k < aff_words / 4; k++) {
+      uint4 v = pp[k];
+      nz |= v.x | v.y | v.z | v.w;
+    }
+    if (!nz)
+      for (int k = 0; k < 8; k++) s[k] = 0;
+  }
   const uint32_t B = 1u << (c - 1);
   uint32_t carry = 0;
   for (int w = 0; w < W; w++) {
@@ -93,15 +126,16 @@ __global__ void k_digits(const uint32_t* __restrict__ scalars, uint32_t n, int c
       d = (int32_t)v;
       carry = 0;
     }
-    uint32_t key = NOKEY, val = 0;
+    uint32_t key = NOKEY, val = 0, rank = 0;
     if (d != 0) {
       uint32_t mag = d < 0 ? (uint32_t)(-d) : (uint32_t)d;
-      key = (uint32_t)w * B + mag - 1;
+      key = key_base + (uint32_t)w * B + mag - 1;
       val = i | (d < 0 ? 0x80000000u : 0u);
-      atomicAdd(&counts[key], 1u);
+      rank = atomicAdd(&counts[key], 1u);
     }
     keys[(uint64_t)w * n + i] = key;
     vals[(uint64_t)w * n + i] = val;
+    ranks[(uint64_t)w * n + i] = rank;
   }
 }
 
@@ -174,15 +208,15 @@ __global__ void k_zero_u32(uint32_t* __restrict__ out, uint64_t n) {
   if (i < n) out[i] = 0;
 }
 
-__global__ void k_scatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, uint64_t total,
-                          uint32_t* __restrict__ cursor, uint32_t* __restrict__ skeys, uint32_t* __restrict__ svals) {
+// sorted[offsets[key] + rank] = (key, val)
+__global__ void k_scatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                          const uint32_t* __restrict__ ranks, uint64_t total, const uint32_t* __restrict__ offsets,
+                          uint2* __restrict__ sorted) {
   uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= total) return;
   uint32_t key = keys[e];
   if (key == NOKEY) return;
-  uint32_t pos = atomicAdd(&cursor[key], 1u);
-  skeys[pos] = key;
-  svals[pos] = vals[e];
+  sorted[offsets[key] + ranks[e]] = make_uint2(key, vals[e]);
 }
 
 __global__ void k_pack16(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint64_t nout) {
@@ -220,14 +254,22 @@ void launch_zero_u32(uint32_t* p, uint64_t n, hipStream_t s) {
 }
 
 // digits + counting sort of (bucket, point) pairs by bucket
-void msm_sort(const uint32_t* d_scalars, uint64_t n, const MsmPlan& p, hipStream_t s, int group, uint32_t* keys,
-              uint32_t* vals, uint32_t* counts, uint32_t* offsets, uint32_t* bsum, uint32_t* cursor, uint32_t* skeys,
-              uint32_t* svals) {
+void msm_sort(const uint32_t* const* d_scalars, const uint32_t* const* d_points, const uint64_t* ns,
+              const MsmPlan& p, hipStream_t s, int group, uint32_t* keys, uint32_t* vals, uint32_t* ranks,
+              uint32_t* counts, uint32_t* offsets, uint32_t* bsum, uint2* sorted, uint32_t* fix, uint64_t fix_words) {
   uint32_t nblk = msm_scan_blocks(p.nb);
   {
       ScopedKTime kt(group == 1 ? "msm_g1_digits" : "msm_g2_digits", s);
       k_zero_u32<<<blocks_for(p.nb + 1, 256), 256, 0, s>>>(counts, (uint64_t)p.nb + 1);
-      k_digits<<<blocks_for(n, 256), 256, 0, s>>>(d_scalars, (uint32_t)n, p.c, p.W, keys, vals, counts);
+      k_zero_u32<<<blocks_for(fix_words, 256), 256, 0, s>>>(fix, fix_words);
+      uint64_t off = 0;
+      for (uint32_t i = 0; i < p.M; i++) {
+        if (ns[i])
+          k_digits<<<blocks_for(ns[i], 256), 256, 0, s>>>(d_scalars[i], d_points[i], group == 1 ? 16 : 32,
+                                                          (uint32_t)ns[i], p.c, p.W, i * p.kspan, keys + off,
+                                                          vals + off, ranks + off, counts);
+        off += (uint64_t)p.W * ns[i];
+      }
       BG_HIP(hipGetLastError());
     }
     {
@@ -235,28 +277,33 @@ void msm_sort(const uint32_t* d_scalars, uint64_t n, const MsmPlan& p, hipStream
       k_scan_local<<<nblk, SCAN_T, 0, s>>>(counts, offsets, bsum, p.nb);
       k_scan_top<<<1, SCAN_T, 0, s>>>(bsum, nblk, offsets + p.nb);
       k_scan_add<<<blocks_for(p.nb, 256), 256, 0, s>>>(offsets, bsum, p.nb);
-      k_copy_u32<<<blocks_for(p.nb, 256), 256, 0, s>>>(offsets, cursor, p.nb);
-      k_scatter<<<blocks_for(p.entries, 256), 256, 0, s>>>(keys, vals, p.entries, cursor, skeys, svals);
+      k_scatter<<<blocks_for(p.entries, 256), 256, 0, s>>>(keys, vals, ranks, p.entries, offsets, sorted);
       BG_HIP(hipGetLastError());
     }
 }
 
+void msm_enqueue_batch(int group, uint32_t M, const uint32_t* const* d_points, const uint32_t* const* d_scalars,
+                       const uint64_t* ns, int window_bits, hipStream_t s, MsmWork& w) {
+  if (group == 1)
+    enqueue_g1(M, d_points, d_scalars, ns, window_bits, s, w);
+  else
+    enqueue_g2(M, d_points, d_scalars, ns, window_bits, s, w);
+}
+
 void msm_enqueue(int group, const uint32_t* d_points, const uint32_t* d_scalars, uint64_t n, int window_bits,
                  hipStream_t s, MsmWork& w) {
-  if (group == 1)
-    enqueue_g1(d_points, d_scalars, n, window_bits, s, w);
-  else
-    enqueue_g2(d_points, d_scalars, n, window_bits, s, w);
+  msm_enqueue_batch(group, 1, &d_points, &d_scalars, &n, window_bits, s, w);
 }
 
 template <class E>
-static bgh::Pt<E> host_finish(const MsmPlan& p, const uint32_t* hout) {
+static void host_finish(const MsmPlan& p, const uint32_t* hout, std::vector<bgh::Pt<E>>& res) {
   using Pt = bgh::Pt<E>;
   const int xw = 4 * bgh::words<E>();
   int log2M = 0;
   while ((1u << log2M) < p.m * p.K) log2M++;
-  std::vector<Pt> wsum(p.W, Pt::inf());
-  auto one_window = [&](int w) {
+  const uint32_t MW = p.M * (uint32_t)p.W;
+  std::vector<Pt> wsum(MW, Pt::inf());
+  auto one_window = [&](uint32_t w) {
     const uint32_t* base = hout + (uint64_t)2 * p.H * xw * w;
     Pt sumP = Pt::inf(), run = Pt::inf(), wq = Pt::inf();
     for (int h = (int)p.H - 1; h >= 0; h--) {
@@ -268,36 +315,48 @@ static bgh::Pt<E> host_finish(const MsmPlan& p, const uint32_t* hout) {
     }
     wsum[w] = sumP.add(wq.dbl_n(log2M));
   };
-  int nth = std::min<int>(p.W, (int)std::max(1u, std::thread::hardware_concurrency()));
-  if (p.H * p.W < 64 || nth <= 1) {
-    for (int w = 0; w < p.W; w++) one_window(w);
+  int nth = std::min<int>((int)MW, (int)std::max(1u, std::thread::hardware_concurrency()));
+  nth = std::min(nth, 32);
+  if ((uint64_t)p.H * MW < 64 || nth <= 1) {
+    for (uint32_t w = 0; w < MW; w++) one_window(w);
   } else {
     std::vector<std::thread> th;
     for (int t = 0; t < nth; t++)
       th.emplace_back([&, t] {
-        for (int w = t; w < p.W; w += nth) one_window(w);
+        for (uint32_t w = t; w < MW; w += nth) one_window(w);
       });
     for (auto& x : th) x.join();
   }
-  Pt acc = Pt::inf();
-  for (int w = p.W - 1; w >= 0; w--) acc = acc.dbl_n(p.c).add(wsum[w]);
-  return acc;
+  res.assign(p.M, Pt::inf());
+  for (uint32_t i = 0; i < p.M; i++) {
+    Pt acc = Pt::inf();
+    for (int w = p.W - 1; w >= 0; w--) acc = acc.dbl_n(p.c).add(wsum[i * p.W + w]);
+    res[i] = acc;
+  }
 }
 
 void msm_finish(MsmWork& w, uint32_t* out_affine, uint32_t* xyzz_out) {
   BG_HIP(hipEventSynchronize(w.done));
   const uint32_t* hout = static_cast<const uint32_t*>(w.host_out.ptr);
   if (w.group == 1) {
-    auto r = host_finish<bgh::Fq>(w.plan, hout);
-    if (out_affine) bgh::store_affine(r, out_affine);
-    if (xyzz_out) {
-      r.X.to_u32(xyzz_out); r.Y.to_u32(xyzz_out + 8); r.ZZ.to_u32(xyzz_out + 16); r.ZZZ.to_u32(xyzz_out + 24);
+    std::vector<bgh::P1> r;
+    host_finish<bgh::Fq>(w.plan, hout, r);
+    for (uint32_t i = 0; i < w.plan.M; i++) {
+      if (out_affine) bgh::store_affine(r[i], out_affine + 16 * i);
+      if (xyzz_out) {
+        uint32_t* o = xyzz_out + 32 * i;
+        r[i].X.to_u32(o); r[i].Y.to_u32(o + 8); r[i].ZZ.to_u32(o + 16); r[i].ZZZ.to_u32(o + 24);
+      }
     }
   } else {
-    auto r = host_finish<bgh::Fq2>(w.plan, hout);
-    if (out_affine) bgh::store_affine(r, out_affine);
-    if (xyzz_out) {
-      r.X.to_u32(xyzz_out); r.Y.to_u32(xyzz_out + 16); r.ZZ.to_u32(xyzz_out + 32); r.ZZZ.to_u32(xyzz_out + 48);
+    std::vector<bgh::P2> r;
+    host_finish<bgh::Fq2>(w.plan, hout, r);
+    for (uint32_t i = 0; i < w.plan.M; i++) {
+      if (out_affine) bgh::store_affine(r[i], out_affine + 32 * i);
+      if (xyzz_out) {
+        uint32_t* o = xyzz_out + 64 * i;
+        r[i].X.to_u32(o); r[i].Y.to_u32(o + 16); r[i].ZZ.to_u32(o + 32); r[i].ZZZ.to_u32(o + 48);
+      }
     }
   }
 }
@@ -354,6 +413,57 @@ static int msm_api(int group, const uint32_t* points, const uint32_t* scalars, u
     } else {
       memcpy(out, res, pw * 4);
     }
+  });
+}
+
+extern "C" int bg_msm_batch(int group, uint32_t count, const uint32_t* const* points, const uint32_t* const* scalars,
+                            const uint64_t* ns, uint32_t* outs, int flags, int window_bits) {
+  return guard([&] {
+    BG_REQUIRE((group == 1 || group == 2) && count >= 1 && points && scalars && ns && outs, BG_ERR_ARG,
+               "bg_msm_batch: bad argument");
+    BG_REQUIRE(window_bits == 0 || (window_bits >= 2 && window_bits <= 22), BG_ERR_ARG,
+               "bg_msm_batch: window_bits %d out of range", window_bits);
+    bool words16 = (flags & 1) == BG_FMT_WORDS16;
+    bool dev = (flags & BG_MEM_DEVICE) != 0;
+    BG_REQUIRE(!words16, BG_ERR_ARG, "bg_msm_batch: inputs must be packed");
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> lk(c.mu);
+    const uint64_t pw = group == 1 ? 16 : 32;
+    std::vector<const uint32_t*> dp(count), ds(count);
+    std::vector<uint64_t> n(ns, ns + count);
+    if (dev) {
+      for (uint32_t i = 0; i < count; i++) {
+        dp[i] = points[i];
+        ds[i] = scalars[i];
+      }
+    } else {
+      uint64_t tot = 0;
+      for (uint32_t i = 0; i < count; i++) tot += n[i];
+      uint32_t* bp = c.scratch[10].as<uint32_t>(std::max<uint64_t>(1, tot * pw));
+      uint32_t* bs = c.scratch[11].as<uint32_t>(std::max<uint64_t>(1, tot * 8));
+      uint64_t off = 0;
+      for (uint32_t i = 0; i < count; i++) {
+        dp[i] = bp + off * pw;
+        ds[i] = bs + off * 8;
+        if (n[i]) {
+          BG_HIP(hipMemcpyAsync(bp + off * pw, points[i], n[i] * pw * 4, hipMemcpyHostToDevice, c.stream));
+          BG_HIP(hipMemcpyAsync(bs + off * 8, scalars[i], n[i] * 32, hipMemcpyHostToDevice, c.stream));
+        }
+        off += n[i];
+      }
+    }
+    static MsmWork work[2];
+    MsmWork& w = work[group - 1];
+    {
+      ScopedKTime kt(group == 1 ? "msm_g1_batch" : "msm_g2_batch", c.stream);
+      msm_enqueue_batch(group, count, dp.data(), ds.data(), n.data(), window_bits, c.stream, w);
+    }
+    std::vector<uint32_t> res(pw * count);
+    msm_finish(w, res.data());
+    if (dev)
+      BG_HIP(hipMemcpy(outs, res.data(), res.size() * 4, hipMemcpyHostToDevice));
+    else
+      memcpy(outs, res.data(), res.size() * 4);
   });
 }
 

@@ -14,27 +14,34 @@
 
 namespace bg {
 
+// Schedule of a batch of M MSMs of one group sharing one window width: the batch is
+// handled as one MSM with M*W windows (window mw belongs to MSM mw / W), so every stage
+// runs once over all MSMs.
 struct MsmPlan {
-  int c = 0;         // window width (bits)
-  int W = 0;         // number of windows, W*c >= 255
-  uint32_t B = 0;    // buckets per window, 2^(c-1)
-  uint32_t nb = 0;   // W*B
-  uint32_t m = 0;    // buckets per level-1 thread
-  uint32_t G = 0;    // level-1 groups per window
-  uint32_t K = 0;    // level-2 block size
-  uint32_t H = 0;    // level-2 outputs per window
-  uint32_t L = 0;    // sorted entries per accumulation thread
-  uint64_t n = 0;
-  uint64_t entries = 0;  // W*n
+  int c = 0;          // window width (bits)
+  int W = 0;          // windows per MSM, W*c >= 255
+  uint32_t M = 1;     // MSMs in the batch
+  uint32_t B = 0;     // buckets per window, 2^(c-1)
+  uint32_t kspan = 0; // buckets per MSM, W*B
+  uint32_t nb = 0;    // M*W*B
+  uint32_t m = 0;     // buckets per level-1 thread
+  uint32_t G = 0;     // level-1 groups per window
+  uint32_t K = 0;     // level-2 block size
+  uint32_t H = 0;     // level-2 outputs per window
+  uint32_t L = 0;     // sorted entries per accumulation thread
+  uint64_t n = 0;     // total points
+  uint64_t entries = 0;  // W * total points
   uint32_t acc_threads = 0;
 };
 
 MsmPlan msm_plan(uint64_t n, int group, int window_bits);
+MsmPlan msm_plan_batch(const uint64_t* ns, uint32_t M, int group, int window_bits);
 
 // Work buffers of one in-flight MSM (reused across calls).
 struct MsmWork {
-  DevBuf keys, vals, counts, skeys, svals, buckets, pkeys, pflags, ppts, lvlS, lvlT, outPQ, scan_tmp, red_scratch;
-  HostBuf host_out;
+  DevBuf keys, vals, ranks, counts, sorted, buckets, pkeys, pflags, ppts, lvlS, lvlT, outPQ, scan_tmp, fix, rfix,
+      ptrs;
+  HostBuf host_out, host_ptrs;
   MsmPlan plan;
   int group = 1;
   hipEvent_t done = nullptr;
@@ -46,8 +53,13 @@ struct MsmWork {
 void msm_enqueue(int group, const uint32_t* d_points, const uint32_t* d_scalars, uint64_t n, int window_bits,
                  hipStream_t s, MsmWork& w);
 
-// Waits for the MSM and finishes it on the host: out = affine result (packed words:
-// G1 16, G2 32).  If xyzz_out is non-null, also writes the XYZZ sum (G1 32, G2 64 words).
+// Batch of M MSMs of one group: points[i] / scalars[i] / ns[i] per MSM (device).
+void msm_enqueue_batch(int group, uint32_t M, const uint32_t* const* d_points, const uint32_t* const* d_scalars,
+                       const uint64_t* ns, int window_bits, hipStream_t s, MsmWork& w);
+
+// Waits for the MSM(s) and finishes them on the host: out = affine results, one per MSM
+// in the batch (packed words: G1 16, G2 32 each).  xyzz_out (optional) receives the XYZZ
+// sums (G1 32, G2 64 words each).
 void msm_finish(MsmWork& w, uint32_t* out_affine, uint32_t* xyzz_out = nullptr);
 
 // Device kernels shared with the prover: FORMAT.md 16-bit words -> packed words.
@@ -58,12 +70,12 @@ void launch_unpack16(const uint32_t* in32, uint32_t* out16, uint64_t in_words, h
 // pipeline pieces shared by the per-group translation units
 uint32_t msm_scan_blocks(uint32_t n);
 void launch_zero_u32(uint32_t* p, uint64_t n, hipStream_t s);
-void msm_sort(const uint32_t* d_scalars, uint64_t n, const MsmPlan& p, hipStream_t s, int group, uint32_t* keys,
-              uint32_t* vals, uint32_t* counts, uint32_t* offsets, uint32_t* bsum, uint32_t* cursor, uint32_t* skeys,
-              uint32_t* svals);
-void enqueue_g1(const uint32_t* d_points, const uint32_t* d_scalars, uint64_t n, int window_bits, hipStream_t s,
-                MsmWork& w);
-void enqueue_g2(const uint32_t* d_points, const uint32_t* d_scalars, uint64_t n, int window_bits, hipStream_t s,
-                MsmWork& w);
+void msm_sort(const uint32_t* const* d_scalars, const uint32_t* const* d_points, const uint64_t* ns,
+              const MsmPlan& p, hipStream_t s, int group, uint32_t* keys, uint32_t* vals, uint32_t* ranks,
+              uint32_t* counts, uint32_t* offsets, uint32_t* bsum, uint2* sorted, uint32_t* fix, uint64_t fix_words);
+void enqueue_g1(uint32_t M, const uint32_t* const* d_points, const uint32_t* const* d_scalars, const uint64_t* ns,
+                int window_bits, hipStream_t s, MsmWork& w);
+void enqueue_g2(uint32_t M, const uint32_t* const* d_points, const uint32_t* const* d_scalars, const uint64_t* ns,
+                int window_bits, hipStream_t s, MsmWork& w);
 
 }  // namespace bg

@@ -14,14 +14,32 @@ static constexpr uint32_t NOKEY_C = 0xFFFFFFFFu;
 // Each thread sums L consecutive sorted entries.  Runs wholly inside the slice are
 // complete buckets; a run touching a slice edge whose key continues into the neighbour
 // is a partial: slot 2t+1 if it continues into the next slice, else slot 2t.
+// The hot loop uses only the generic mixed-add formula (every run starts from its first
+// point and no addend is infinity); a run that meets x(acc) = x(point) is flagged, its
+// bucket key is listed once in fix[1 + nb ..] (count in fix[0], per-bucket flag in
+// fix[1 ..]) and k_fixup recomputes that bucket exactly.
 template <class F>
-__global__ __launch_bounds__(256) void k_accumulate(const uint32_t* __restrict__ points,
-                                                    const uint32_t* __restrict__ skeys,
-                                                    const uint32_t* __restrict__ svals,
+__device__ __forceinline__ void flag_bucket(uint32_t key, uint32_t nb, uint32_t* __restrict__ fix) {
+  if (atomicOr(&fix[1 + key], 1u) == 0u) {
+    uint32_t idx = atomicAdd(&fix[0], 1u);
+    fix[1 + nb + idx] = key;
+  }
+}
+
+// Point base of an entry: the batch's MSM is key / kspan.
+__device__ __forceinline__ const uint32_t* entry_points(const uint64_t* __restrict__ ptrs, uint32_t key,
+                                                        uint32_t kspan) {
+  return reinterpret_cast<const uint32_t*>(ptrs[key / kspan]);
+}
+
+template <class F>
+__global__ __launch_bounds__(256, 2) void k_accumulate(const uint64_t* __restrict__ ptrs, uint32_t kspan,
+                                                    const uint2* __restrict__ sorted,
                                                     const uint32_t* __restrict__ total_ptr, uint32_t L,
                                                     uint32_t nthreads, uint32_t* __restrict__ buckets,
                                                     uint32_t* __restrict__ pkeys, uint32_t* __restrict__ pflags,
-                                                    uint32_t* __restrict__ ppts) {
+                                                    uint32_t* __restrict__ ppts, uint32_t nb,
+                                                    uint32_t* __restrict__ fix) {
   using C = Curve<F>;
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nthreads) return;
@@ -31,15 +49,24 @@ __global__ __launch_bounds__(256) void k_accumulate(const uint32_t* __restrict__
   uint64_t start = (uint64_t)t * L;
   if (start >= T) return;
   uint64_t end = min(start + L, (uint64_t)T);
-  uint32_t prevkey = start > 0 ? skeys[start - 1] : NOKEY_C;
-  uint32_t nextkey = end < T ? skeys[end] : NOKEY_C;
-  uint32_t cur = skeys[start];
+  uint32_t prevkey = start > 0 ? sorted[start - 1].x : NOKEY_C;
+  uint32_t nextkey = end < T ? sorted[end].x : NOKEY_C;
+  uint2 e0 = sorted[start];
+  uint32_t cur = e0.x;
+  const uint32_t* points = entry_points(ptrs, cur, kspan);
+  typename C::A p0 = C::load_aff(points + (uint64_t)C::aff_words * (e0.y & 0x7FFFFFFFu));
+  if (e0.y >> 31) p0.y = F::neg(p0.y);
+  typename C::P acc = {p0.x, p0.y, F::one(), F::one()};
   bool first = true;
-  typename C::P acc = C::infinity();
-  for (uint64_t e = start; e < end; e++) {
-    uint32_t k = skeys[e];
-    if (k != cur) {
+  uint32_t bad = 0;
+  for (uint64_t e = start + 1; e < end; e++) {
+    uint2 ent = sorted[e];
+    if (ent.x / kspan != cur / kspan) points = entry_points(ptrs, ent.x, kspan);
+    typename C::A pt = C::load_aff(points + (uint64_t)C::aff_words * (ent.y & 0x7FFFFFFFu));
+    if (ent.y >> 31) pt.y = F::neg(pt.y);
+    if (ent.x != cur) {
       bool cp = first && prevkey == cur;
+      if (bad) flag_bucket<F>(cur, nb, fix);
       if (!cp) {
         C::store(buckets + (uint64_t)C::xyzz_words * cur, acc);
       } else {
@@ -48,16 +75,16 @@ __global__ __launch_bounds__(256) void k_accumulate(const uint32_t* __restrict__
         C::store(ppts + (uint64_t)C::xyzz_words * (2 * t), acc);
       }
       first = false;
-      cur = k;
-      acc = C::infinity();
+      cur = ent.x;
+      bad = 0;
+      acc = {pt.x, pt.y, F::one(), F::one()};
+    } else {
+      acc = C::madd_generic(acc, pt, bad);
     }
-    uint32_t v = svals[e];
-    typename C::A pt = C::load_aff(points + (uint64_t)C::aff_words * (v & 0x7FFFFFFFu));
-    if (v >> 31) pt = C::aff_neg(pt);
-    acc = C::madd(acc, pt);
   }
   bool cp = first && prevkey == cur;
   bool cn = nextkey == cur;
+  if (bad) flag_bucket<F>(cur, nb, fix);
   if (!cp && !cn) {
     C::store(buckets + (uint64_t)C::xyzz_words * cur, acc);
   } else {
@@ -65,6 +92,29 @@ __global__ __launch_bounds__(256) void k_accumulate(const uint32_t* __restrict__
     pkeys[slot] = cur;
     pflags[slot] = cp ? 1u : 0u;
     C::store(ppts + (uint64_t)C::xyzz_words * slot, acc);
+  }
+}
+
+// Exact recomputation of the flagged buckets (doubling / cancellation inside a bucket:
+// repeated or opposite bases), with the special-case-complete mixed add.
+template <class F>
+__global__ __launch_bounds__(64) void k_fixup(const uint64_t* __restrict__ ptrs, uint32_t kspan,
+                                              const uint2* __restrict__ sorted,
+                                              const uint32_t* __restrict__ offsets, uint32_t nb,
+                                              const uint32_t* __restrict__ fix, uint32_t* __restrict__ buckets) {
+  using C = Curve<F>;
+  uint32_t count = fix[0];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+    uint32_t key = fix[1 + nb + i];
+    const uint32_t* points = entry_points(ptrs, key, kspan);
+    typename C::P acc = C::infinity();
+    for (uint32_t e = offsets[key]; e < offsets[key + 1]; e++) {
+      uint32_t v = sorted[e].y;
+      typename C::A pt = C::load_aff(points + (uint64_t)C::aff_words * (v & 0x7FFFFFFFu));
+      if (v >> 31) pt = C::aff_neg(pt);
+      acc = C::madd(acc, pt);
+    }
+    C::store(buckets + (uint64_t)C::xyzz_words * key, acc);
   }
 }
 
@@ -95,81 +145,108 @@ __global__ __launch_bounds__(256) void k_merge(const uint32_t* __restrict__ pkey
 
 // Level 1: thread g sums buckets i = j*m + l (l < m) of window w:
 //   S = sum_l b_l,  T = sum_l l * b_l   (running sums from the top)
-template <class F>
+// CAREFUL = false uses the generic add with infinity selected around it and lists the
+// group in rl[1 ..] (count rl[0]) when two partial sums share an x coordinate;
+// CAREFUL = true recomputes only the listed groups with the exact add.
+template <class F, bool CAREFUL>
 __global__ __launch_bounds__(256) void k_reduce1(const uint32_t* __restrict__ buckets,
                                                  const uint32_t* __restrict__ offsets, uint32_t B, uint32_t m,
                                                  uint32_t ngroups, uint32_t* __restrict__ lvlS,
-                                                 uint32_t* __restrict__ lvlT) {
+                                                 uint32_t* __restrict__ lvlT, uint32_t* __restrict__ rl) {
   using C = Curve<F>;
-  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= ngroups) return;
-  uint32_t G = B / m;
-  uint32_t w = g / G, j = g % G;
-  uint64_t base = (uint64_t)w * B + (uint64_t)j * m;
-  typename C::P R = C::infinity(), T = C::infinity();
-  for (int l = (int)m - 1; l >= 0; l--) {
-    uint64_t key = base + l;
-    if (offsets[key + 1] > offsets[key]) R = C::add(R, C::load(buckets + (uint64_t)C::xyzz_words * key));
-    if (l > 0) T = C::add(T, R);
+  uint32_t count = CAREFUL ? rl[0] : ngroups;
+  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < count; idx += gridDim.x * blockDim.x) {
+    uint32_t g = CAREFUL ? rl[1 + idx] : idx;
+    uint32_t G = B / m;
+    uint32_t w = g / G, j = g % G;
+    uint64_t base = (uint64_t)w * B + (uint64_t)j * m;
+    typename C::P R = C::infinity(), T = C::infinity();
+    uint32_t bad = 0;
+    for (int l = (int)m - 1; l >= 0; l--) {
+      uint64_t key = base + l;
+      if (offsets[key + 1] > offsets[key]) {
+        typename C::P b = C::load(buckets + (uint64_t)C::xyzz_words * key);
+        R = CAREFUL ? C::add(R, b) : C::add_lite(R, b, bad);
+      }
+      if (l > 0) T = CAREFUL ? C::add(T, R) : C::add_lite(T, R, bad);
+    }
+    C::store(lvlS + (uint64_t)C::xyzz_words * g, R);
+    C::store(lvlT + (uint64_t)C::xyzz_words * g, T);
+    if (!CAREFUL && bad) rl[1 + atomicAdd(&rl[0], 1u)] = g;
   }
-  C::store(lvlS + (uint64_t)C::xyzz_words * g, R);
-  C::store(lvlT + (uint64_t)C::xyzz_words * g, T);
 }
 
 // Level 2: a block of K = blockDim.x threads (power of two) takes K consecutive level-1
 // groups of one window:
 //   U = sum_k (T_k + S_k),  V = sum_k k*S_k (suffix scan + tree),  Z = sum_k S_k
 //   out P = U + m*V, Q = Z
-template <class F>
+// CAREFUL as in k_reduce1, per block (list in rl2).
+template <class F, bool CAREFUL>
 __global__ __launch_bounds__(256) void k_reduce2(const uint32_t* __restrict__ lvlS, const uint32_t* __restrict__ lvlT,
-                                                 int log2m, uint32_t* __restrict__ outPQ) {
+                                                 int log2m, uint32_t* __restrict__ outPQ, uint32_t* __restrict__ rl2) {
   using C = Curve<F>;
   using P = typename C::P;
   extern __shared__ __align__(16) unsigned char red2_smem[];
   const uint32_t K = blockDim.x;
   P* shA = reinterpret_cast<P*>(red2_smem);
   P* shU = shA + K;
-  uint32_t k = threadIdx.x;
-  uint64_t g = (uint64_t)blockIdx.x * K + k;
-  P S = C::load(lvlS + (uint64_t)C::xyzz_words * g);
-  P T = C::load(lvlT + (uint64_t)C::xyzz_words * g);
-  P a = S;
-  shA[k] = a;
-  shU[k] = C::add(T, S);
-  __syncthreads();
-  for (uint32_t d = 1; d < K; d <<= 1) {
-    P x = (k + d < K) ? shA[k + d] : C::infinity();
-    __syncthreads();
-    a = C::add(a, x);
+  __shared__ uint32_t blk_bad;
+  uint32_t count = CAREFUL ? rl2[0] : gridDim.x;
+  for (uint32_t bi = blockIdx.x; bi < count; bi += gridDim.x) {
+    uint32_t blk = CAREFUL ? rl2[1 + bi] : bi;
+    uint32_t k = threadIdx.x;
+    uint32_t bad = 0;
+    if (k == 0) blk_bad = 0;
+    uint64_t g = (uint64_t)blk * K + k;
+    P S = C::load(lvlS + (uint64_t)C::xyzz_words * g);
+    P T = C::load(lvlT + (uint64_t)C::xyzz_words * g);
+    P a = S;
     shA[k] = a;
+    shU[k] = CAREFUL ? C::add(T, S) : C::add_lite(T, S, bad);
     __syncthreads();
-  }
-  P Z = shA[0];
-  __syncthreads();
-  if (k == 0) shA[0] = C::infinity();
-  __syncthreads();
-  for (uint32_t s = K / 2; s > 0; s >>= 1) {
-    if (k < s) {
-      shA[k] = C::add(shA[k], shA[k + s]);
-      shU[k] = C::add(shU[k], shU[k + s]);
+    for (uint32_t d = 1; d < K; d <<= 1) {
+      P x = (k + d < K) ? shA[k + d] : C::infinity();
+      __syncthreads();
+      a = CAREFUL ? C::add(a, x) : C::add_lite(a, x, bad);
+      shA[k] = a;
+      __syncthreads();
     }
+    P Z = shA[0];
     __syncthreads();
-  }
-  if (k == 0) {
-    P V = shA[0];
-    for (int i = 0; i < log2m; i++) V = C::dbl(V);
-    P out = C::add(shU[0], V);
-    C::store(outPQ + (uint64_t)C::xyzz_words * (2 * blockIdx.x), out);
-    C::store(outPQ + (uint64_t)C::xyzz_words * (2 * blockIdx.x + 1), Z);
+    if (k == 0) shA[0] = C::infinity();
+    __syncthreads();
+    for (uint32_t s = K / 2; s > 0; s >>= 1) {
+      if (k < s) {
+        shA[k] = CAREFUL ? C::add(shA[k], shA[k + s]) : C::add_lite(shA[k], shA[k + s], bad);
+        shU[k] = CAREFUL ? C::add(shU[k], shU[k + s]) : C::add_lite(shU[k], shU[k + s], bad);
+      }
+      __syncthreads();
+    }
+    if (k == 0) {
+      P V = shA[0];
+      for (int i = 0; i < log2m; i++) V = C::dbl(V);
+      P out = CAREFUL ? C::add(shU[0], V) : C::add_lite(shU[0], V, bad);
+      C::store(outPQ + (uint64_t)C::xyzz_words * (2 * blk), out);
+      C::store(outPQ + (uint64_t)C::xyzz_words * (2 * blk + 1), Z);
+    }
+    if (!CAREFUL && bad) atomicOr(&blk_bad, 1u);
+    __syncthreads();
+    if (!CAREFUL && k == 0 && blk_bad) rl2[1 + atomicAdd(&rl2[0], 1u)] = blk;
+    __syncthreads();
   }
 }
 
 template <class F>
-void enqueue_curve(const uint32_t* d_points, const uint32_t* d_scalars, uint64_t n, int window_bits,
-                         hipStream_t s, MsmWork& w, int group) {
+void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* const* d_scalars, const uint64_t* ns,
+                   int window_bits, hipStream_t s, MsmWork& w, int group) {
   using C = Curve<F>;
-  BG_REQUIRE(n < (1ull << 31), BG_ERR_ARG, "msm: n = %llu exceeds 2^31 - 1", (unsigned long long)n);
-  MsmPlan p = msm_plan(n, group, window_bits);
+  BG_REQUIRE(M >= 1, BG_ERR_ARG, "msm: empty batch");
+  for (uint32_t i = 0; i < M; i++)
+    BG_REQUIRE(ns[i] < (1ull << 31), BG_ERR_ARG, "msm: n = %llu exceeds 2^31 - 1", (unsigned long long)ns[i]);
+  MsmPlan p = msm_plan_batch(ns, M, group, window_bits);
+  const uint64_t n = p.n;
+  BG_REQUIRE(p.entries < (1ull << 32) && (uint64_t)M * p.kspan < (1ull << 31), BG_ERR_ARG,
+             "msm: batch too large (%llu entries)", (unsigned long long)p.entries);
   BG_REQUIRE(p.c >= 2 && p.c <= 22, BG_ERR_ARG, "msm: window_bits %d out of range", p.c);
   w.plan = p;
   w.group = group;
@@ -178,20 +255,26 @@ void enqueue_curve(const uint32_t* d_points, const uint32_t* d_scalars, uint64_t
   uint32_t* vals = w.vals.as<uint32_t>(p.entries);
   uint32_t* counts = w.counts.as<uint32_t>(2 * ((uint64_t)p.nb + 1));  // counts | offsets
   uint32_t* offsets = counts + p.nb + 1;
+  uint32_t* ranks = w.ranks.as<uint32_t>(p.entries);
   uint32_t nblk = msm_scan_blocks(p.nb);
-  uint32_t* tmp = w.scan_tmp.as<uint32_t>((uint64_t)nblk + 8 + p.nb);
-  uint32_t* bsum = tmp;
-  uint32_t* cursor = tmp + nblk + 8;
-  uint32_t* skeys = w.skeys.as<uint32_t>(p.entries);
-  uint32_t* svals = w.svals.as<uint32_t>(p.entries);
+  uint32_t* bsum = w.scan_tmp.as<uint32_t>((uint64_t)nblk + 8);
+  uint2* sorted = w.sorted.as<uint2>(p.entries);
+  uint32_t* fix = w.fix.as<uint32_t>(1 + 2ull * p.nb);  // count | per-bucket flags | key list
+  // reduction exception lists: level-1 count | groups, level-2 count | blocks
+  uint64_t ngr = (uint64_t)p.M * p.W * p.G, nblk2 = ngr / p.K;
+  uint32_t* rl1 = w.rfix.as<uint32_t>(2 + ngr + nblk2);
+  uint32_t* rl2 = rl1 + 1 + ngr;
   uint32_t* buckets = w.buckets.as<uint32_t>((uint64_t)p.nb * xw);
   uint32_t* pkeys = w.pkeys.as<uint32_t>(2ull * p.acc_threads);
   uint32_t* pflags = w.pflags.as<uint32_t>(2ull * p.acc_threads);
   uint32_t* ppts = w.ppts.as<uint32_t>(2ull * p.acc_threads * xw);
-  uint64_t ngroups = (uint64_t)p.W * p.G;
+  uint64_t ngroups = (uint64_t)p.M * p.W * p.G;
   uint32_t* lvlS = w.lvlS.as<uint32_t>(ngroups * xw);
   uint32_t* lvlT = w.lvlT.as<uint32_t>(ngroups * xw);
-  uint64_t nout = (uint64_t)p.W * p.H;
+  uint64_t nout = (uint64_t)p.M * p.W * p.H;
+  uint64_t* hptrs = static_cast<uint64_t*>(w.host_ptrs.get(8ull * M));
+  for (uint32_t i = 0; i < M; i++) hptrs[i] = reinterpret_cast<uint64_t>(d_points[i]);
+  uint64_t* dptrs = w.ptrs.as<uint64_t>(M);
   uint32_t* outPQ = w.outPQ.as<uint32_t>(2 * nout * xw);
   uint32_t* hout = static_cast<uint32_t*>(w.host_out.get(2 * nout * xw * 4));
 
@@ -199,23 +282,35 @@ void enqueue_curve(const uint32_t* d_points, const uint32_t* d_scalars, uint64_t
     // empty MSM: every bucket empty -> zero outputs
     launch_zero_u32(outPQ, 2 * nout * xw, s);
   } else {
-    msm_sort(d_scalars, n, p, s, group, keys, vals, counts, offsets, bsum, cursor, skeys, svals);
+    BG_HIP(hipMemcpyAsync(dptrs, hptrs, 8ull * M, hipMemcpyHostToDevice, s));
+    msm_sort(d_scalars, d_points, ns, p, s, group, keys, vals, ranks, counts, offsets, bsum, sorted, fix,
+             1 + (uint64_t)p.nb);
     {
       ScopedKTime kt(group == 1 ? "msm_g1_accumulate" : "msm_g2_accumulate", s);
-      k_accumulate<F><<<blocks_of(p.acc_threads, 256), 256, 0, s>>>(d_points, skeys, svals, offsets + p.nb, p.L,
-                                                                     p.acc_threads, buckets, pkeys, pflags, ppts);
+      k_accumulate<F><<<blocks_of(p.acc_threads, 256), 256, 0, s>>>(dptrs, p.kspan, sorted, offsets + p.nb, p.L,
+                                                                     p.acc_threads, buckets, pkeys, pflags, ppts,
+                                                                     p.nb, fix);
       k_merge<F><<<blocks_of(p.acc_threads, 256), 256, 0, s>>>(pkeys, pflags, ppts, p.acc_threads, buckets);
+      k_fixup<F><<<4, 64, 0, s>>>(dptrs, p.kspan, sorted, offsets, p.nb, fix, buckets);
       BG_HIP(hipGetLastError());
     }
     {
-      ScopedKTime kt(group == 1 ? "msm_g1_reduce" : "msm_g2_reduce", s);
-      k_reduce1<F><<<blocks_of(ngroups, 256), 256, 0, s>>>(buckets, offsets, p.B, p.m, (uint32_t)ngroups, lvlS, lvlT);
       int log2m = 0;
       while ((1u << log2m) < p.m) log2m++;
       unsigned nblocks = (unsigned)(ngroups / p.K);
       size_t smem = 2 * (size_t)p.K * sizeof(typename C::P);
       BG_REQUIRE(smem <= 65536, BG_ERR_INTERNAL, "msm: level-2 block %u needs %zu B of LDS", p.K, smem);
-      k_reduce2<F><<<nblocks, p.K, smem, s>>>(lvlS, lvlT, log2m, outPQ);
+      launch_zero_u32(rl1, 1, s);
+      launch_zero_u32(rl2, 1, s);
+      {
+        ScopedKTime kt(group == 1 ? "msm_g1_reduce1" : "msm_g2_reduce1", s);
+        k_reduce1<F, false><<<blocks_of(ngroups, 256), 256, 0, s>>>(buckets, offsets, p.B, p.m, (uint32_t)ngroups,
+                                                                     lvlS, lvlT, rl1);
+        k_reduce1<F, true><<<16, 64, 0, s>>>(buckets, offsets, p.B, p.m, (uint32_t)ngroups, lvlS, lvlT, rl1);
+      }
+      ScopedKTime kt(group == 1 ? "msm_g1_reduce2" : "msm_g2_reduce2", s);
+      k_reduce2<F, false><<<nblocks, p.K, smem, s>>>(lvlS, lvlT, log2m, outPQ, rl2);
+      k_reduce2<F, true><<<8, p.K, smem, s>>>(lvlS, lvlT, log2m, outPQ, rl2);
       BG_HIP(hipGetLastError());
     }
   }

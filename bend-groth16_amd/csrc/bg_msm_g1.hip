@@ -2,8 +2,8 @@
 #include "bg_msm_curve.hpp"
 
 namespace bg {
-void enqueue_g1(const uint32_t* d_points, const uint32_t* d_scalars, uint64_t n, int window_bits, hipStream_t s,
-                MsmWork& w) {
-  enqueue_curve<Fq>(d_points, d_scalars, n, window_bits, s, w, 1);
+void enqueue_g1(uint32_t M, const uint32_t* const* d_points, const uint32_t* const* d_scalars, const uint64_t* ns,
+                int window_bits, hipStream_t s, MsmWork& w) {
+  enqueue_curve<Fq>(M, d_points, d_scalars, ns, window_bits, s, w, 1);
 }
 }  // namespace bg

@@ -2,8 +2,8 @@
 #include "bg_msm_curve.hpp"
 
 namespace bg {
-void enqueue_g2(const uint32_t* d_points, const uint32_t* d_scalars, uint64_t n, int window_bits, hipStream_t s,
-                MsmWork& w) {
-  enqueue_curve<Fq2>(d_points, d_scalars, n, window_bits, s, w, 2);
+void enqueue_g2(uint32_t M, const uint32_t* const* d_points, const uint32_t* const* d_scalars, const uint64_t* ns,
+                int window_bits, hipStream_t s, MsmWork& w) {
+  enqueue_curve<Fq2>(M, d_points, d_scalars, ns, window_bits, s, w, 2);
 }
 }  // namespace bg

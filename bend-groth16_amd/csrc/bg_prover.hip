@@ -114,6 +114,15 @@ __global__ void k_from_mont(const uint32_t* __restrict__ in, uint32_t* __restric
 
 static unsigned nb(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
+// MSM schedule inside one proof (see bg_prove); BG16_PROVE_SCHEDULE=0|1 selects it.
+static int prove_schedule() {
+  static int s = [] {
+    const char* e = getenv("BG16_PROVE_SCHEDULE");
+    return e ? atoi(e) : 0;
+  }();
+  return s;
+}
+
 // ------------------------------------------------------------------ host combination
 
 
@@ -225,8 +234,8 @@ extern "C" int bg_pk_free(bg_pk* pk) {
     std::lock_guard<std::mutex> lk(ctx().mu);
     for (auto& w : pk->work) {
       if (w.done) (void)hipEventDestroy(w.done);
-      for (DevBuf* b : {&w.keys, &w.vals, &w.counts, &w.skeys, &w.svals, &w.buckets, &w.pkeys, &w.pflags, &w.ppts,
-                        &w.lvlS, &w.lvlT, &w.outPQ, &w.scan_tmp, &w.red_scratch})
+      for (DevBuf* b : {&w.keys, &w.vals, &w.ranks, &w.counts, &w.sorted, &w.buckets, &w.pkeys, &w.pflags, &w.ppts,
+                        &w.lvlS, &w.lvlT, &w.outPQ, &w.scan_tmp, &w.fix, &w.rfix})
         b->release();
       if (w.host_out.ptr) (void)hipHostFree(w.host_out.ptr);
     }
@@ -262,53 +271,73 @@ extern "C" int bg_prove(bg_pk* pk, const uint32_t* witness, const uint32_t* rs, 
     const uint32_t nc = pk->hdr[0], ni = pk->hdr[1], nw = pk->hdr[2], lg = pk->hdr[3];
     const uint32_t na = pk->hdr[4], nbq = pk->hdr[5], nh = pk->hdr[6], nl = pk->hdr[7];
     const uint64_t d = 1ull << lg, nz = (uint64_t)ni + nw;
-    hipStream_t s = c.stream;
-    hipEvent_t ev[9];
+    // Streams: s0 runs the witness path (upload, evaluation vectors, canonical scalars),
+    // qap_h and then MSM H; MSMs A+L, B1 and B2 each run on their own stream once the
+    // canonical scalars exist, so the latency-bound tails of one MSM overlap with the
+    // accumulation of the others.
+    hipStream_t s0 = c.stream, s1 = c.aux[0], s2 = c.aux[1], s3 = c.aux[2];
+    hipEvent_t ev[16];
     for (auto& e : ev) BG_HIP(hipEventCreate(&e));
-    BG_HIP(hipEventRecord(ev[0], s));
+    BG_HIP(hipEventRecord(ev[0], s0));
     // 1. witness upload (16-bit words are packed on the device)
     uint32_t* z = pk->z.as<uint32_t>();
     if (dev_witness) {
-      BG_HIP(hipMemcpyAsync(z, witness, 32 * nz, hipMemcpyDeviceToDevice, s));
+      BG_HIP(hipMemcpyAsync(z, witness, 32 * nz, hipMemcpyDeviceToDevice, s0));
     } else if (words16) {
-      BG_HIP(hipMemcpyAsync(pk->zraw.ptr, witness, 64 * nz, hipMemcpyHostToDevice, s));
-      launch_pack16(pk->zraw.as<uint32_t>(), z, 8 * nz, s);
+      BG_HIP(hipMemcpyAsync(pk->zraw.ptr, witness, 64 * nz, hipMemcpyHostToDevice, s0));
+      launch_pack16(pk->zraw.as<uint32_t>(), z, 8 * nz, s0);
     } else {
-      BG_HIP(hipMemcpyAsync(z, witness, 32 * nz, hipMemcpyHostToDevice, s));
+      BG_HIP(hipMemcpyAsync(z, witness, 32 * nz, hipMemcpyHostToDevice, s0));
     }
-    BG_HIP(hipEventRecord(ev[1], s));
+    BG_HIP(hipEventRecord(ev[1], s0));
     // 2. evaluation vectors and canonical scalars
     uint32_t* zc = pk->zc.as<uint32_t>();
     {
-      ScopedKTime kt("evals", s);
-      k_evals<<<nb(d, 256), 256, 0, s>>>(pk->A.offs.as<uint32_t>(), pk->A.cols.as<uint32_t>(), pk->A.coef.as<uint32_t>(),
-                                         z, nc, ni, 1, d, pk->ea.as<uint32_t>());
-      k_evals<<<nb(d, 256), 256, 0, s>>>(pk->B.offs.as<uint32_t>(), pk->B.cols.as<uint32_t>(), pk->B.coef.as<uint32_t>(),
-                                         z, nc, ni, 0, d, pk->eb.as<uint32_t>());
-      k_evals<<<nb(d, 256), 256, 0, s>>>(pk->C.offs.as<uint32_t>(), pk->C.cols.as<uint32_t>(), pk->C.coef.as<uint32_t>(),
-                                         z, nc, ni, 0, d, pk->ec.as<uint32_t>());
-      k_from_mont<<<nb(nz, 256), 256, 0, s>>>(z, zc, nz);
+      ScopedKTime kt("evals", s0);
+      k_from_mont<<<nb(nz, 256), 256, 0, s0>>>(z, zc, nz);
+      BG_HIP(hipEventRecord(ev[2], s0));  // canonical scalars ready
+      k_evals<<<nb(d, 256), 256, 0, s0>>>(pk->A.offs.as<uint32_t>(), pk->A.cols.as<uint32_t>(),
+                                          pk->A.coef.as<uint32_t>(), z, nc, ni, 1, d, pk->ea.as<uint32_t>());
+      k_evals<<<nb(d, 256), 256, 0, s0>>>(pk->B.offs.as<uint32_t>(), pk->B.cols.as<uint32_t>(),
+                                          pk->B.coef.as<uint32_t>(), z, nc, ni, 0, d, pk->eb.as<uint32_t>());
+      k_evals<<<nb(d, 256), 256, 0, s0>>>(pk->C.offs.as<uint32_t>(), pk->C.cols.as<uint32_t>(),
+                                          pk->C.coef.as<uint32_t>(), z, nc, ni, 0, d, pk->ec.as<uint32_t>());
       BG_HIP(hipGetLastError());
     }
-    BG_HIP(hipEventRecord(ev[2], s));
-    // 3. h (canonical, the MSM H scalars)
-    {
-      ScopedKTime kt("qap_h", s);
-      qap_h_run(pk->ea.as<uint32_t>(), pk->eb.as<uint32_t>(), pk->ec.as<uint32_t>(), pk->h.as<uint32_t>(),
-                pk->tmp.as<uint32_t>(), (int)lg, true, s);
+    BG_HIP(hipEventRecord(ev[3], s0));
+    // 3. MSMs.  B2 (G2) starts on its own stream as soon as the canonical scalars exist.
+    //    The G1 MSMs run as one batch (one sort, accumulation and reduction over all their
+    //    windows): either A, B1, L right away on s1 with H alone after qap_h (schedule 0),
+    //    or A, B1, L, H together after qap_h on s0 (schedule 1).
+    const int sched = prove_schedule();
+    for (hipStream_t sx : {s1, s2, s3}) BG_HIP(hipStreamWaitEvent(sx, ev[2], 0));
+    BG_HIP(hipEventRecord(ev[9], s3));
+    msm_enqueue(2, pk->pb2.as<uint32_t>(), zc, nbq, 0, s3, pk->work[2]);
+    BG_HIP(hipEventRecord(ev[10], s3));
+    const uint32_t* g1p[4] = {pk->pa.as<uint32_t>(), pk->pb1.as<uint32_t>(), pk->pl.as<uint32_t>(),
+                              pk->ph.as<uint32_t>()};
+    const uint32_t* g1s[4] = {zc, zc, zc + 8 * ni, pk->h.as<uint32_t>()};
+    const uint64_t g1n[4] = {na, nbq, nl, nh};
+    if (sched == 0) {
+      BG_HIP(hipEventRecord(ev[5], s1));
+      msm_enqueue_batch(1, 3, g1p, g1s, g1n, 0, s1, pk->work[0]);
+      BG_HIP(hipEventRecord(ev[6], s1));
     }
-    BG_HIP(hipEventRecord(ev[3], s));
-    // 4. MSMs in the reference order: A, B1, B2, H, L (prove.bend:64-116)
-    msm_enqueue(1, pk->pa.as<uint32_t>(), zc, na, 0, s, pk->work[0]);
-    BG_HIP(hipEventRecord(ev[4], s));
-    msm_enqueue(1, pk->pb1.as<uint32_t>(), zc, nbq, 0, s, pk->work[1]);
-    BG_HIP(hipEventRecord(ev[5], s));
-    msm_enqueue(2, pk->pb2.as<uint32_t>(), zc, nbq, 0, s, pk->work[2]);
-    BG_HIP(hipEventRecord(ev[6], s));
-    msm_enqueue(1, pk->ph.as<uint32_t>(), pk->h.as<uint32_t>(), nh, 0, s, pk->work[3]);
-    BG_HIP(hipEventRecord(ev[7], s));
-    msm_enqueue(1, pk->pl.as<uint32_t>(), zc + 8 * ni, nl, 0, s, pk->work[4]);
-    BG_HIP(hipEventRecord(ev[8], s));
+    // 4. h (canonical, the MSM H scalars), then the H MSM (or the whole G1 batch)
+    {
+      ScopedKTime kt("qap_h", s0);
+      qap_h_run(pk->ea.as<uint32_t>(), pk->eb.as<uint32_t>(), pk->ec.as<uint32_t>(), pk->h.as<uint32_t>(),
+                pk->tmp.as<uint32_t>(), (int)lg, true, s0);
+    }
+    BG_HIP(hipEventRecord(ev[4], s0));
+    if (sched == 0) {
+      msm_enqueue(1, pk->ph.as<uint32_t>(), pk->h.as<uint32_t>(), nh, 0, s0, pk->work[1]);
+    } else {
+      BG_HIP(hipEventRecord(ev[5], s0));
+      msm_enqueue_batch(1, 4, g1p, g1s, g1n, 0, s0, pk->work[0]);
+      BG_HIP(hipEventRecord(ev[6], s0));
+    }
+    BG_HIP(hipEventRecord(ev[11], s0));
 
     // host work that depends only on r, s overlaps with the device
     uint32_t rs_packed[16];
@@ -330,17 +359,20 @@ extern "C" int bg_prove(bg_pk* pk, const uint32_t* witness, const uint32_t* rs, 
     bgh::P1 r_d1 = delta1.mul(rc), s_d1 = delta1.mul(sc), rs_d1 = delta1.mul(rsc);
     bgh::P2 s_d2 = delta2.mul(sc);
 
-    // 5. finish the MSMs and combine (groth16.bend:298-308)
-    uint32_t mA[16], mB1[16], mB2[32], mH[16], mL[16];
-    msm_finish(pk->work[0], mA);
-    msm_finish(pk->work[1], mB1);
+    // 5. finish the MSMs and combine (groth16.bend:298-308); slots as prove.bend:64-116
+    uint32_t g1res[4 * 16], mB2[32], mHs[16];
+    msm_finish(pk->work[0], g1res);
+    const uint32_t *mA = g1res, *mB1 = g1res + 16, *mL = g1res + 32;
+    const uint32_t* mH = g1res + 48;
     bgh::P1 A = bgh::load_affine<bgh::Fq>(mA).add(r_d1).add(alpha);
     bgh::P1 B1 = bgh::load_affine<bgh::Fq>(mB1).add(s_d1).add(beta1);
     bgh::P1 sA_rB = A.mul(sc).add(B1.mul(rc));
     msm_finish(pk->work[2], mB2);
     bgh::P2 B2 = bgh::load_affine<bgh::Fq2>(mB2).add(s_d2).add(beta2);
-    msm_finish(pk->work[3], mH);
-    msm_finish(pk->work[4], mL);
+    if (sched == 0) {
+      msm_finish(pk->work[1], mHs);
+      mH = mHs;
+    }
     bgh::P1 C = sA_rB.add(rs_d1.neg()).add(bgh::load_affine<bgh::Fq>(mL)).add(bgh::load_affine<bgh::Fq>(mH));
     uint32_t proof[64];
     bgh::store_affine(A, proof);
@@ -356,11 +388,13 @@ extern "C" int bg_prove(bg_pk* pk, const uint32_t* witness, const uint32_t* rs, 
     }
     auto t1 = std::chrono::steady_clock::now();
     if (t_ms) {
-      // device time of each phase (one stream, so the phases run back to back), then
-      // the wall time of the whole call
+      // device time of each phase on its own stream (the MSMs overlap), then the wall
+      // time of the whole call
+      // upload, evals, qap, G1 batch, (unused), B2, H alone (schedule 0) / G1 batch end
+      const int pairs[8][2] = {{0, 1}, {1, 3}, {3, 4}, {5, 6}, {5, 6}, {9, 10}, {4, 11}, {5, 6}};
       for (int i = 0; i < 8; i++) {
         float e = 0;
-        BG_HIP(hipEventElapsedTime(&e, ev[i], ev[i + 1]));
+        BG_HIP(hipEventElapsedTime(&e, ev[pairs[i][0]], ev[pairs[i][1]]));
         t_ms[i] = e;
       }
       t_ms[8] = std::chrono::duration<double, std::milli>(t1 - t0).count();

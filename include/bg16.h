@@ -72,6 +72,13 @@ int bg_msm_g1(const uint32_t* points, const uint32_t* scalars, uint64_t n, uint3
               int flags, int window_bits);
 int bg_msm_g2(const uint32_t* points, const uint32_t* scalars, uint64_t n, uint32_t* out,
               int flags, int window_bits);
+/* count MSMs of one group in one launch sequence (Msm.g1 / Msm.g2 applied to count
+ * independent inputs of uneven sizes): every stage -- digit decomposition, bucket sort,
+ * accumulation, bucket reduction -- runs once over the whole batch.  points[i],
+ * scalars[i]: ns[i] packed points / canonical scalars (host, or device with
+ * BG_MEM_DEVICE); outs: count packed affine results (host, or device). */
+int bg_msm_batch(int group, uint32_t count, const uint32_t* const* points, const uint32_t* const* scalars,
+                 const uint64_t* ns, uint32_t* outs, int flags, int window_bits);
 /* The MSM schedule chosen for n points of group (1 G1, 2 G2): info8 = window bits c,
  * windows W, buckets per window, sorted entries (W n), entries per accumulation thread,
  * accumulation threads, level-1 groups, level-2 outputs.  No counterpart in the
@@ -111,7 +118,8 @@ int bg_pk_device_bytes(const bg_pk* pk, uint64_t* bytes);
  * proof_out (host): A (G1), B (G2), C (G1) affine Montgomery (the proof_ref.bin layout),
  * in the flags' encoding.
  * t_ms (optional, 9 doubles): device time of upload, evals, qap, msm_a, msm_b1, msm_b2,
- * msm_h, msm_l (the T phases of bend2/prove.bend:154-162), then the call's wall time. */
+ * msm_h, msm_l (the T phases of bend2/prove.bend:154-162; the MSMs run concurrently on
+ * separate streams, so they overlap), then the call's wall time. */
 int bg_prove(bg_pk* pk, const uint32_t* witness, const uint32_t* rs, uint32_t* proof_out,
              int flags, double* t_ms);
 

@@ -122,3 +122,57 @@ def test_msm_closed_form(bg, group, log2n):
     want = _mul_gen(group, O.fr_dot(sc, ks))
     print(f"msm closed form G{group} 2^{log2n}", flush=True)
     assert np.array_equal(_msm(group, pts, sc), want)
+
+
+@pytest.mark.parametrize("group", [1, 2], ids=["G1", "G2"])
+def test_msm_repeated_and_opposite_bases(bg, group):
+    """Buckets whose generic-formula accumulation meets x(acc) = x(P): the same base many
+    times (doublings, spanning many accumulation slices) and P / -P pairs (cancellation)
+    mixed with ordinary entries; exact recomputation of those buckets must give the sum."""
+    rng = np.random.default_rng(77 + group)
+    n = 2500
+    pts, ks = _bases(group, rng, n)
+    sc = O.rand_scalars(rng, n)
+    pts[100:1300] = pts[99]          # 1201 copies of one base ...
+    sc[100:1300] = sc[99]            # ... with one scalar: heavy buckets of equal points
+    half = pts.shape[1] // 2
+    for i in range(1400, 1420):      # opposite bases with equal scalars
+        y = O.array_to_ints(pts[i + 40, half:].reshape(-1, 4), None)
+        pts[i, :half] = pts[i + 40, :half]
+        pts[i, half:] = O.ints_to_array([(O.P_Q - O.unmont(v, O.P_Q)) % O.P_Q for v in y], O.P_Q).reshape(-1)
+        sc[i] = sc[i + 40]
+    assert np.array_equal(_msm(group, pts, sc), _oracle(group, pts, sc))
+
+
+@pytest.mark.parametrize("group", [1, 2], ids=["G1", "G2"])
+def test_msm_batch_uneven(bg, group):
+    """Independent MSMs of uneven sizes (including empty and single-point ones) in one
+    batched launch sequence equal each MSM computed alone."""
+    rng = np.random.default_rng(300 + group)
+    sizes = [1, 0, 37, 1000, 5, 4096, 2] if group == 1 else [3, 0, 500, 1, 64]
+    pts_l, sc_l, want = [], [], []
+    for n in sizes:
+        pts, _ = _bases(group, rng, max(n, 1))
+        pts = pts[:n]
+        sc = O.rand_scalars(rng, n)
+        pts_l.append(_u32(pts))
+        sc_l.append(_u32(sc))
+        want.append(_oracle(group, pts, sc) if n else np.zeros(8 if group == 1 else 16, np.uint64))
+    got = bendg.msm_batch(group, pts_l, sc_l).view(np.uint64)
+    for i, n in enumerate(sizes):
+        assert np.array_equal(got[i], want[i]), f"MSM {i} (n={n})"
+
+
+def test_msm_batch_closed_form_large(bg):
+    rng = np.random.default_rng(400)
+    sizes = [1 << 12, 3 << 14, 1 << 16, 70001]
+    pts_l, sc_l, want = [], [], []
+    for n in sizes:
+        pts, ks = _bases(1, rng, n)
+        sc = O.rand_scalars(rng, n)
+        pts_l.append(_u32(pts))
+        sc_l.append(_u32(sc))
+        want.append(_mul_gen(1, O.fr_dot(sc, ks)))
+    got = bendg.msm_batch(1, pts_l, sc_l).view(np.uint64)
+    for i in range(len(sizes)):
+        assert np.array_equal(got[i], want[i])
