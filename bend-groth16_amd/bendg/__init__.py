@@ -21,7 +21,8 @@ from pathlib import Path
 import numpy as np
 
 PKG_DIR = Path(__file__).resolve().parent.parent
-LIB_PATH = PKG_DIR / "libbg16.so"
+# BG16_LIB selects another build of the library (A/B timing of two builds)
+LIB_PATH = Path(os.environ["BG16_LIB"]) if os.environ.get("BG16_LIB") else PKG_DIR / "libbg16.so"
 
 FMT_WORDS16 = 0
 FMT_PACKED = 1

@@ -88,6 +88,36 @@ __device__ __forceinline__ uint32_t get_bits(const uint32_t* s, int lo, int c) {
   return v & ((1u << c) - 1u);
 }
 
+// Rank of this lane's entry inside bucket `key` (NOKEY: no entry), from the histogram
+// atomic.  Lanes of a wave that share the key of the first pending lane take a single
+// atomic between them (small or repeated scalars, or a short top window, send many
+// digits of a wave to one bucket); once the first pending key is unshared the rest take
+// one atomic each, so distinct keys cost one extra ballot.
+__device__ __forceinline__ uint32_t bucket_rank(uint32_t* __restrict__ counts, uint32_t key) {
+  const uint32_t lane = __lane_id();
+  const uint64_t below = (1ull << lane) - 1ull;
+  bool pending = key != NOKEY;
+  uint32_t rank = 0;
+  for (int round = 0; round < 8; round++) {
+    uint64_t pend = __ballot(pending);
+    if (!pend) break;
+    int leader = __ffsll((unsigned long long)pend) - 1;
+    uint32_t lkey = __shfl(key, leader);
+    bool mine = pending && key == lkey;
+    uint64_t same = __ballot(mine);
+    if (__popcll(same) == 1) break;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(&counts[lkey], (uint32_t)__popcll(same));
+    base = __shfl(base, leader);
+    if (mine) {
+      rank = base + (uint32_t)__popcll(same & below);
+      pending = false;
+    }
+  }
+  if (pending) rank = atomicAdd(&counts[key], 1u);
+  return rank;
+}
+
 // Signed c-bit digits of every scalar; for each nonzero digit the (bucket key, point
 // index | sign) pair and its rank inside the bucket (the value the histogram atomic
 // returns), so the scatter needs no atomics.
@@ -126,13 +156,13 @@ __global__ void k_digits(const uint32_t* __restrict__ scalars, const uint32_t* _
       d = (int32_t)v;
       carry = 0;
     }
-    uint32_t key = NOKEY, val = 0, rank = 0;
+    uint32_t key = NOKEY, val = 0;
     if (d != 0) {
       uint32_t mag = d < 0 ? (uint32_t)(-d) : (uint32_t)d;
       key = key_base + (uint32_t)w * B + mag - 1;
       val = i | (d < 0 ? 0x80000000u : 0u);
-      rank = atomicAdd(&counts[key], 1u);
     }
+    uint32_t rank = bucket_rank(counts, key);
     keys[(uint64_t)w * n + i] = key;
     vals[(uint64_t)w * n + i] = val;
     ranks[(uint64_t)w * n + i] = rank;
@@ -315,24 +345,21 @@ static void host_finish(const MsmPlan& p, const uint32_t* hout, std::vector<bgh:
     }
     wsum[w] = sumP.add(wq.dbl_n(log2M));
   };
-  int nth = std::min<int>((int)MW, (int)std::max(1u, std::thread::hardware_concurrency()));
-  nth = std::min(nth, 32);
-  if ((uint64_t)p.H * MW < 64 || nth <= 1) {
+  if ((uint64_t)p.H * MW < 64) {
     for (uint32_t w = 0; w < MW; w++) one_window(w);
   } else {
-    std::vector<std::thread> th;
-    for (int t = 0; t < nth; t++)
-      th.emplace_back([&, t] {
-        for (uint32_t w = t; w < MW; w += nth) one_window(w);
-      });
-    for (auto& x : th) x.join();
+    host_parallel((int)MW, [&](int w) { one_window((uint32_t)w); });
   }
   res.assign(p.M, Pt::inf());
-  for (uint32_t i = 0; i < p.M; i++) {
+  auto horner = [&](int i) {
     Pt acc = Pt::inf();
     for (int w = p.W - 1; w >= 0; w--) acc = acc.dbl_n(p.c).add(wsum[i * p.W + w]);
     res[i] = acc;
-  }
+  };
+  if (p.M == 1)
+    horner(0);
+  else
+    host_parallel((int)p.M, horner);
 }
 
 void msm_finish(MsmWork& w, uint32_t* out_affine, uint32_t* xyzz_out) {

@@ -39,12 +39,23 @@ MsmPlan msm_plan_batch(const uint64_t* ns, uint32_t M, int group, int window_bit
 
 // Work buffers of one in-flight MSM (reused across calls).
 struct MsmWork {
-  DevBuf keys, vals, ranks, counts, sorted, buckets, pkeys, pflags, ppts, lvlS, lvlT, outPQ, scan_tmp, fix, rfix,
-      ptrs;
+  DevBuf keys, vals, ranks, counts, sorted, buckets, pkeys, pflags, ppts, mkeys, mflags, mpts, lvlS, lvlT, outPQ,
+      scan_tmp, fix, rfix, ptrs;
   HostBuf host_out, host_ptrs;
   MsmPlan plan;
   int group = 1;
   hipEvent_t done = nullptr;
+  hipEvent_t mid = nullptr;  // recorded once the buckets are complete (before the reductions)
+  void release() {
+    for (DevBuf* b : {&keys, &vals, &ranks, &counts, &sorted, &buckets, &pkeys, &pflags, &ppts, &mkeys, &mflags,
+                      &mpts, &lvlS, &lvlT, &outPQ, &scan_tmp, &fix, &rfix, &ptrs})
+      b->release();
+    host_out.release();
+    host_ptrs.release();
+    if (done) (void)hipEventDestroy(done);
+    if (mid) (void)hipEventDestroy(mid);
+    done = mid = nullptr;
+  }
 };
 
 // Enqueue the device part of an MSM on stream s: points (packed affine, device),

@@ -118,29 +118,108 @@ __global__ __launch_bounds__(64) void k_fixup(const uint64_t* __restrict__ ptrs,
   }
 }
 
-// A group of partials for one bucket starts at a tail slot whose run does not continue
-// from the previous slice; it spans following slices' 'both' tails and ends at a head.
+// Partial runs, one item per accumulation thread (and, one level up, per merge block):
+//   head slot 2t   : run that continues from item t-1 and ends in item t (NOKEY if none)
+//   tail slot 2t+1 : run that continues into item t+1; flag 1 when it also continues from
+//                    item t-1 (the run covers the whole item), else it starts in item t.
+// A bucket spanning items t0..t1 is tail(t0) + tail(t0+1 .. t1-1) + head(t1).
+// One merge level: a block of MERGE_T items sums each run of tail slots with a
+// segmented tree in LDS (log2(MERGE_T) rounds of at most one add), adds the closing head
+// and writes the finished bucket; a run entering from the previous block or leaving into
+// the next becomes the block's head / tail slot of the next level, in the same format.
+// Levels repeat until one block covers all items, so a bucket spread over m items costs
+// O(log m) adds of depth instead of a serial walk (scalars that share digits -- small or
+// repeated witness values, a short top window -- make such buckets).
+// MERGE_T items per block: 256 for G1 (32 KB of LDS points), 128 for G2 (32 KB).
 template <class F>
-__global__ __launch_bounds__(256) void k_merge(const uint32_t* __restrict__ pkeys, const uint32_t* __restrict__ pflags,
-                                               const uint32_t* __restrict__ ppts, uint32_t nthreads,
-                                               uint32_t* __restrict__ buckets) {
+struct MergeT {
+  static constexpr uint32_t value = sizeof(typename Curve<F>::P) > 128 ? 128u : 256u;
+};
+
+template <class F, uint32_t MERGE_T = MergeT<F>::value>
+__global__ __launch_bounds__(MERGE_T) void k_merge_level(const uint32_t* __restrict__ ikeys,
+                                                     const uint32_t* __restrict__ iflags,
+                                                     const uint32_t* __restrict__ ipts, uint32_t nitems,
+                                                     uint32_t* __restrict__ buckets, uint32_t* __restrict__ okeys,
+                                                     uint32_t* __restrict__ oflags, uint32_t* __restrict__ opts) {
   using C = Curve<F>;
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nthreads) return;
-  uint32_t slot = 2 * t + 1;
-  uint32_t k = pkeys[slot];
-  if (k == NOKEY_C || pflags[slot]) return;
-  typename C::P acc = C::load(ppts + (uint64_t)C::xyzz_words * slot);
-  for (uint32_t u = t + 1; u < nthreads; u++) {
-    uint32_t s2 = 2 * u + 1;
-    if (pkeys[s2] == k && pflags[s2]) {
-      acc = C::add(acc, C::load(ppts + (uint64_t)C::xyzz_words * s2));
-      continue;
-    }
-    if (pkeys[2 * u] == k) acc = C::add(acc, C::load(ppts + (uint64_t)C::xyzz_words * (2 * u)));
-    break;
+  using P = typename C::P;
+  extern __shared__ __align__(16) unsigned char merge_smem[];
+  P* shV = reinterpret_cast<P*>(merge_smem);
+  __shared__ uint32_t shK[MERGE_T], shF[MERGE_T];
+  __shared__ int shS[MERGE_T], shE[MERGE_T];
+  const uint32_t i = threadIdx.x;
+  const uint32_t b = blockIdx.x;
+  const uint64_t u = (uint64_t)b * MERGE_T + i;
+  const uint64_t xw = C::xyzz_words;
+  uint32_t tk = NOKEY_C, tf = 0;
+  if (u < nitems) {
+    tk = ikeys[2 * u + 1];
+    tf = tk != NOKEY_C ? iflags[2 * u + 1] : 0u;
   }
-  C::store(buckets + (uint64_t)C::xyzz_words * k, acc);
+  shK[i] = tk;
+  shF[i] = tf;
+  P v;
+  if (tk != NOKEY_C) {
+    v = C::load(ipts + xw * (2 * u + 1));
+    shV[i] = v;
+  }
+  __syncthreads();
+  // segment bounds among tail slots: start = nearest start marker at or below i,
+  // end = nearest end marker at or above i
+  bool used = tk != NOKEY_C;
+  int sm = (used && (tf == 0 || i == 0)) ? (int)i : -1;
+  bool cont_next = i + 1 < MERGE_T && shK[i + 1] != NOKEY_C && shF[i + 1] != 0;
+  int em = (used && !cont_next) ? (int)i : (int)MERGE_T;
+  shS[i] = sm;
+  shE[i] = em;
+  __syncthreads();
+  for (uint32_t d = 1; d < MERGE_T; d <<= 1) {
+    int a = i >= d ? shS[i - d] : -1;
+    int c = i + d < MERGE_T ? shE[i + d] : (int)MERGE_T;
+    __syncthreads();
+    sm = max(sm, a);
+    em = min(em, c);
+    shS[i] = sm;
+    shE[i] = em;
+    __syncthreads();
+  }
+  const int st = sm, en = em;
+  const uint32_t rel = used ? i - (uint32_t)st : 1u;
+  const uint32_t len = used ? (uint32_t)(en - st + 1) : 0u;
+  for (uint32_t d = 1; d < MERGE_T; d <<= 1) {
+    if (used && (rel & (2 * d - 1)) == 0 && rel + d < len) {
+      v = C::add(v, shV[i + d]);
+      shV[i] = v;
+    }
+    __syncthreads();
+  }
+  if (i == 0) {
+    // head slot of this block at the next level: a run from the previous block that
+    // ends here with no tail slot in this block passes through unchanged
+    uint32_t hk = u < nitems ? ikeys[2 * u] : NOKEY_C;
+    okeys[2 * b] = hk;
+    if (hk != NOKEY_C) C::store(opts + xw * (2 * b), C::load(ipts + xw * (2 * u)));
+  }
+  if (i == MERGE_T - 1 && !used) okeys[2 * b + 1] = NOKEY_C;
+  if (used && rel == 0) {
+    const bool from_prev = i == 0 && tf != 0;
+    if ((uint32_t)en + 1 < MERGE_T) {
+      // the run closes at item u_end + 1 inside this block
+      uint64_t uh = (uint64_t)b * MERGE_T + (uint32_t)en + 1;
+      v = C::add(v, C::load(ipts + xw * (2 * uh)));
+      if (from_prev) {
+        okeys[2 * b] = tk;
+        C::store(opts + xw * (2 * b), v);
+      } else {
+        C::store(buckets + xw * tk, v);
+      }
+    } else {
+      okeys[2 * b + 1] = tk;
+      oflags[2 * b + 1] = from_prev ? 1u : 0u;
+      C::store(opts + xw * (2 * b + 1), v);
+    }
+  }
 }
 
 // Level 1: thread g sums buckets i = j*m + l (l < m) of window w:
@@ -268,6 +347,11 @@ void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* 
   uint32_t* pkeys = w.pkeys.as<uint32_t>(2ull * p.acc_threads);
   uint32_t* pflags = w.pflags.as<uint32_t>(2ull * p.acc_threads);
   uint32_t* ppts = w.ppts.as<uint32_t>(2ull * p.acc_threads * xw);
+  // merge levels ping-pong between two halves of mcap items each
+  const uint64_t mcap = blocks_of(p.acc_threads, MergeT<F>::value);
+  uint32_t* mkeys = w.mkeys.as<uint32_t>(4 * mcap);
+  uint32_t* mflags = w.mflags.as<uint32_t>(4 * mcap);
+  uint32_t* mpts = w.mpts.as<uint32_t>(4 * mcap * xw);
   uint64_t ngroups = (uint64_t)p.M * p.W * p.G;
   uint32_t* lvlS = w.lvlS.as<uint32_t>(ngroups * xw);
   uint32_t* lvlT = w.lvlT.as<uint32_t>(ngroups * xw);
@@ -290,10 +374,32 @@ void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* 
       k_accumulate<F><<<blocks_of(p.acc_threads, 256), 256, 0, s>>>(dptrs, p.kspan, sorted, offsets + p.nb, p.L,
                                                                      p.acc_threads, buckets, pkeys, pflags, ppts,
                                                                      p.nb, fix);
-      k_merge<F><<<blocks_of(p.acc_threads, 256), 256, 0, s>>>(pkeys, pflags, ppts, p.acc_threads, buckets);
+      {
+        const uint32_t* ik = pkeys;
+        const uint32_t* ifl = pflags;
+        const uint32_t* ip = ppts;
+        uint32_t items = p.acc_threads;
+        int side = 0;
+        constexpr uint32_t MERGE_T = MergeT<F>::value;
+        size_t msmem = MERGE_T * sizeof(typename C::P);
+        do {
+          uint32_t blocks = blocks_of(items, MERGE_T);
+          uint32_t* ok = mkeys + (uint64_t)side * 2 * mcap;
+          uint32_t* of = mflags + (uint64_t)side * 2 * mcap;
+          uint32_t* op = mpts + (uint64_t)side * 2 * mcap * xw;
+          k_merge_level<F><<<blocks, MERGE_T, msmem, s>>>(ik, ifl, ip, items, buckets, ok, of, op);
+          ik = ok;
+          ifl = of;
+          ip = op;
+          items = blocks;
+          side ^= 1;
+        } while (items > 1);
+      }
       k_fixup<F><<<4, 64, 0, s>>>(dptrs, p.kspan, sorted, offsets, p.nb, fix, buckets);
       BG_HIP(hipGetLastError());
     }
+    if (!w.mid) BG_HIP(hipEventCreateWithFlags(&w.mid, hipEventDisableTiming));
+    BG_HIP(hipEventRecord(w.mid, s));
     {
       int log2m = 0;
       while ((1u << log2m) < p.m) log2m++;

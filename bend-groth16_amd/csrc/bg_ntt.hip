@@ -1,10 +1,12 @@
 // Radix-2 NTT over BN254 Fr on gfx950 (C ABI: bg_ntt, bg_qap_h).  See bg_ntt.hpp.
 //
-// A transform is a bit-reversal gather (optionally fused with a coset / n^-1 scale)
-// followed by log2 n radix-2 DIT stages: the first TILE_LOG2 stages of every 2^TILE_LOG2
-// contiguous block run in LDS (one kernel), the remaining stages as global passes.
-// Output is the DFT in natural order (bend2/gen/ref_qap.py:27-54 computes the same
-// transform with the same omega = root^(2^(28 - log2))).
+// A transform is a short list of passes; each pass runs up to NTT_TILE_LOG2 radix-2
+// stages of one workgroup's elements in LDS (one global read and write per pass).
+// Decimation in frequency takes natural order to bit-reversed order and decimation in
+// time takes bit-reversed order back to natural order, so qap_h chains inverse (DIF) and
+// coset (DIT) transforms with no reordering pass between them; a final gather puts h
+// into natural order.  A 2^19 transform is two passes.  Results equal the DFT that
+// bend2/gen/ref_qap.py:27-54 computes, with the same omega = root^(2^(28 - log2)).
 #include "bg_ntt.hpp"
 #include "bg_msm.hpp"
 #include "bg_field.hpp"
@@ -14,7 +16,7 @@
 
 namespace bg {
 
-static constexpr int TILE_LOG2 = 10;
+static constexpr int NTT_TILE_LOG2 = 10;   // elements per workgroup: 1024 (32 KB of LDS)
 static constexpr int NTT_THREADS = 256;
 
 __global__ void k_geom(uint32_t* __restrict__ out, uint64_t n, const uint32_t* __restrict__ lo,
@@ -28,65 +30,75 @@ __global__ void k_geom(uint32_t* __restrict__ out, uint64_t n, const uint32_t* _
 
 __device__ __forceinline__ uint32_t bitrev(uint32_t i, int log2) { return log2 ? (__brev(i) >> (32 - log2)) : 0; }
 
-// out[rev(i)] = in[i] * scale[i] (scale optional)
-__global__ void k_bitrev(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, int log2,
-                         const uint32_t* __restrict__ scale) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (1u << log2)) return;
-  Fp x = load_fp(in + 8ull * i);
-  if (scale) x = Fr::mul(x, load_fp(scale + 8ull * i));
-  store_fp(out + 8ull * bitrev(i, log2), x);
+// out[k] = in[rev(k)] * (scale ? scale[k] : c)   (bit-reversed order -> natural order)
+__global__ void k_gather(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, int log2,
+                         const uint32_t* __restrict__ scale, Fp c, int use_c) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (1u << log2)) return;
+  Fp x = load_fp(in + 8ull * bitrev(k, log2));
+  if (scale)
+    x = Fr::mul(x, load_fp(scale + 8ull * k));
+  else if (use_c)
+    x = Fr::mul(x, c);
+  store_fp(out + 8ull * k, x);
 }
 
-// the first min(log2, TILE_LOG2) DIT stages of each contiguous tile, in LDS
-__global__ __launch_bounds__(NTT_THREADS) void k_ntt_tile(uint32_t* __restrict__ d, int log2,
-                                                          const uint32_t* __restrict__ tw) {
-  __shared__ Fp sh[1 << TILE_LOG2];
-  const int tl = log2 < TILE_LOG2 ? log2 : TILE_LOG2;
-  const uint32_t tile = 1u << tl;
-  const uint64_t base = (uint64_t)blockIdx.x * tile;
-  for (uint32_t i = threadIdx.x; i < tile; i += NTT_THREADS) sh[i] = load_fp(d + 8 * (base + i));
+// One pass: radix-2 stages k = k0 .. k0+t-1 (half-size m = 2^(log2-1-k)).  The stages
+// only couple elements whose indices differ by multiples of S = 2^(log2-k0-t) inside an
+// aligned block of 2^(log2-k0) elements, so a workgroup takes the 2^t elements
+// base + j*S + q (j < 2^t) for each of C = 2^logC consecutive offsets q (C consecutive
+// elements per row keep the global accesses coalesced).
+//   DIF: k ascending,  (u, v) -> (u + v, (u - v) w)
+//   DIT: k descending, (u, v) -> (u + v w, u - v w)
+// with w = omega^((i mod m) * 2^k), i the upper element's index, from the n/2-entry
+// table tw.  scale (optional) multiplies each element by scale[index] as it is loaded.
+// src and dst may alias (in-place pass).
+template <bool DIT>
+__global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass(const uint32_t* src, uint32_t* dst, int log2, int k0, int t,
+                                                          int logC, const uint32_t* __restrict__ tw,
+                                                          const uint32_t* __restrict__ scale) {
+  __shared__ Fp sh[1 << NTT_TILE_LOG2];
+  const int S_log = log2 - k0 - t;
+  const uint32_t C = 1u << logC;
+  const uint32_t qgroups = 1u << (S_log - logC);
+  const uint64_t blk = blockIdx.x >> (S_log - logC);
+  const uint64_t q0 = (uint64_t)(blockIdx.x & (qgroups - 1)) << logC;
+  const uint64_t base = (blk << (log2 - k0)) + q0;
+  const uint32_t cnt = 1u << (t + logC);
+  for (uint32_t e = threadIdx.x; e < cnt; e += NTT_THREADS) {
+    uint64_t idx = base + ((uint64_t)(e >> logC) << S_log) + (e & (C - 1));
+    Fp x = load_fp(src + 8 * idx);
+    if (scale) x = Fr::mul(x, load_fp(scale + 8 * idx));
+    sh[e] = x;
+  }
   __syncthreads();
-  for (int st = 0; st < tl; st++) {
-    const uint32_t m = 1u << st;
-    const int tws = log2 - 1 - st;  // twiddle stride n / (2m) = 2^(log2 - 1 - st)
-    for (uint32_t b = threadIdx.x; b < tile / 2; b += NTT_THREADS) {
-      uint32_t j = b & (m - 1);
-      uint32_t i0 = ((b >> st) << (st + 1)) + j;
-      uint32_t i1 = i0 + m;
-      Fp w = load_fp(tw + 8ull * ((uint64_t)j << tws));
-      Fp u = sh[i0];
-      Fp v = Fr::mul(sh[i1], w);
-      sh[i0] = Fr::add(u, v);
-      sh[i1] = Fr::sub(u, v);
+  const uint32_t nbf = cnt >> 1;
+  for (int s = 0; s < t; s++) {
+    const int k = DIT ? k0 + t - 1 - s : k0 + s;
+    const int lmr = log2 - 1 - k - S_log;  // local row distance m / S = 2^lmr
+    for (uint32_t b = threadIdx.x; b < nbf; b += NTT_THREADS) {
+      uint32_t c = b & (C - 1), bj = b >> logC;
+      uint32_t jl = bj & ((1u << lmr) - 1);
+      uint32_t j0 = ((bj >> lmr) << (lmr + 1)) + jl;
+      uint32_t e0 = (j0 << logC) + c, e1 = e0 + (1u << (lmr + logC));
+      uint64_t ex = ((((uint64_t)jl) << S_log) + q0 + c) << k;
+      Fp w = load_fp(tw + 8 * ex);
+      Fp u = sh[e0], v = sh[e1];
+      if (DIT) {
+        v = Fr::mul(v, w);
+        sh[e0] = Fr::add(u, v);
+        sh[e1] = Fr::sub(u, v);
+      } else {
+        sh[e0] = Fr::add(u, v);
+        sh[e1] = Fr::mul(Fr::sub(u, v), w);
+      }
     }
     __syncthreads();
   }
-  for (uint32_t i = threadIdx.x; i < tile; i += NTT_THREADS) store_fp(d + 8 * (base + i), sh[i]);
-}
-
-// one global DIT stage with half-size m = 2^st
-__global__ void k_ntt_stage(uint32_t* __restrict__ d, int log2, int st, const uint32_t* __restrict__ tw) {
-  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= (1u << (log2 - 1))) return;
-  const uint32_t m = 1u << st;
-  uint32_t j = b & (m - 1);
-  uint64_t i0 = ((uint64_t)(b >> st) << (st + 1)) + j;
-  uint64_t i1 = i0 + m;
-  Fp w = load_fp(tw + 8ull * ((uint64_t)j << (log2 - 1 - st)));
-  Fp u = load_fp(d + 8 * i0);
-  Fp v = Fr::mul(load_fp(d + 8 * i1), w);
-  store_fp(d + 8 * i0, Fr::add(u, v));
-  store_fp(d + 8 * i1, Fr::sub(u, v));
-}
-
-// out[i] = in[i] * (table ? table[i] : c)
-__global__ void k_scale(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint64_t n,
-                        const uint32_t* __restrict__ table, Fp c) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  Fp x = load_fp(in + 8 * i);
-  store_fp(out + 8 * i, Fr::mul(x, table ? load_fp(table + 8 * i) : c));
+  for (uint32_t e = threadIdx.x; e < cnt; e += NTT_THREADS) {
+    uint64_t idx = base + ((uint64_t)(e >> logC) << S_log) + (e & (C - 1));
+    store_fp(dst + 8 * idx, sh[e]);
+  }
 }
 
 // a[i] = (a[i] * b[i] - c[i]) * zinv  (ntt.bend:325-379, Ntt.point)
@@ -163,6 +175,12 @@ NttDomain& ntt_domain(int log2) {
   build_geom(dom->sc_ninv_g, n, g, ninv, false, s);
   build_geom(dom->sc_ninv_gi, n, gi, ninv, false, s);
   build_geom(dom->sc_ninv_gi_c, n, gi, ninv, true, s);
+  // n^-1 g^rev(k): the coset scale for coefficients held in bit-reversed order
+  dom->sc_ninv_g_rev.alloc(n * 32);
+  k_gather<<<nblk(n, 256), 256, 0, s>>>(dom->sc_ninv_g.as<uint32_t>(), dom->sc_ninv_g_rev.as<uint32_t>(), log2,
+                                        nullptr, Fp{}, 0);
+  BG_HIP(hipGetLastError());
+  BG_HIP(hipStreamSynchronize(s));
   ninv.to_u32(dom->ninv);
   // zinv = (g^n - 1)^-1: the vanishing polynomial x^n - 1 on the coset g H
   uint64_t e4[4] = {n, 0, 0, 0};
@@ -178,49 +196,93 @@ static Fp to_fp(const uint32_t* w) {
   return r;
 }
 
-static void core(uint32_t* d, int log2, const uint32_t* tw, hipStream_t s) {
-  uint64_t n = 1ull << log2;
-  int tl = log2 < TILE_LOG2 ? log2 : TILE_LOG2;
-  k_ntt_tile<<<(unsigned)(n >> tl), NTT_THREADS, 0, s>>>(d, log2, tw);
-  for (int st = tl; st < log2; st++) k_ntt_stage<<<nblk(n / 2, 256), 256, 0, s>>>(d, log2, st, tw);
+// Pass list of a 2^log2 transform in DIF order (DIT runs it backwards): strided passes
+// of at most NTT_TILE_LOG2 - 1 stages, then one contiguous pass of NTT_TILE_LOG2 stages.
+struct NttPass {
+  int k0, t, logC;
+};
+static std::vector<NttPass> ntt_plan(int log2) {
+  std::vector<NttPass> out;
+  if (log2 <= NTT_TILE_LOG2) {
+    out.push_back({0, log2, 0});
+    return out;
+  }
+  const int rem = log2 - NTT_TILE_LOG2;
+  const int np = (rem + NTT_TILE_LOG2 - 2) / (NTT_TILE_LOG2 - 1);
+  int k0 = 0;
+  for (int p = 0; p < np; p++) {
+    int t = rem / np + (p < rem % np ? 1 : 0);
+    int S_log = log2 - k0 - t;
+    out.push_back({k0, t, std::min(NTT_TILE_LOG2 - t, S_log)});
+    k0 += t;
+  }
+  out.push_back({k0, NTT_TILE_LOG2, 0});
+  return out;
+}
+
+static void launch_pass(bool dit, const uint32_t* src, uint32_t* dst, int log2, const NttPass& p,
+                        const uint32_t* tw, const uint32_t* scale, hipStream_t s) {
+  unsigned wgs = 1u << (log2 - p.t - p.logC);
+  if (dit)
+    k_ntt_pass<true><<<wgs, NTT_THREADS, 0, s>>>(src, dst, log2, p.k0, p.t, p.logC, tw, scale);
+  else
+    k_ntt_pass<false><<<wgs, NTT_THREADS, 0, s>>>(src, dst, log2, p.k0, p.t, p.logC, tw, scale);
+  BG_HIP(hipGetLastError());
+}
+
+// natural order in -> bit-reversed order out; scale multiplies the (natural) input
+static void dif(const uint32_t* src, uint32_t* dst, int log2, const uint32_t* tw, const uint32_t* scale,
+                hipStream_t s) {
+  auto plan = ntt_plan(log2);
+  for (size_t i = 0; i < plan.size(); i++)
+    launch_pass(false, i == 0 ? src : dst, dst, log2, plan[i], tw, i == 0 ? scale : nullptr, s);
+}
+
+// bit-reversed order in -> natural order out; scale multiplies the (bit-reversed) input
+static void dit(const uint32_t* src, uint32_t* dst, int log2, const uint32_t* tw, const uint32_t* scale,
+                hipStream_t s) {
+  auto plan = ntt_plan(log2);
+  for (size_t i = plan.size(); i-- > 0;)
+    launch_pass(true, i + 1 == plan.size() ? src : dst, dst, log2, plan[i], tw,
+                i + 1 == plan.size() ? scale : nullptr, s);
+}
+
+static void gather(const uint32_t* in, uint32_t* out, int log2, const uint32_t* scale, const uint32_t* c,
+                   hipStream_t s) {
+  Fp cf{};
+  if (c) cf = to_fp(c);
+  k_gather<<<nblk(1ull << log2, 256), 256, 0, s>>>(in, out, log2, scale, cf, c ? 1 : 0);
   BG_HIP(hipGetLastError());
 }
 
 void ntt_run(uint32_t* d, uint32_t* tmp, int log2, int direction, hipStream_t s) {
   NttDomain& D = ntt_domain(log2);
-  uint64_t n = D.n;
   bool inv = direction == BG_NTT_IFFT || direction == BG_NTT_COSET_IFFT;
   const uint32_t* pre = direction == BG_NTT_COSET_FFT ? D.sc_g.as<uint32_t>() : nullptr;
-  k_bitrev<<<nblk(n, 256), 256, 0, s>>>(d, tmp, log2, pre);
-  core(tmp, log2, inv ? D.tw_inv.as<uint32_t>() : D.tw_fwd.as<uint32_t>(), s);
+  dif(d, tmp, log2, inv ? D.tw_inv.as<uint32_t>() : D.tw_fwd.as<uint32_t>(), pre, s);
   if (direction == BG_NTT_IFFT)
-    k_scale<<<nblk(n, 256), 256, 0, s>>>(tmp, d, n, nullptr, to_fp(D.ninv));
+    gather(tmp, d, log2, nullptr, D.ninv, s);
   else if (direction == BG_NTT_COSET_IFFT)
-    k_scale<<<nblk(n, 256), 256, 0, s>>>(tmp, d, n, D.sc_ninv_gi.as<uint32_t>(), Fp{});
+    gather(tmp, d, log2, D.sc_ninv_gi.as<uint32_t>(), nullptr, s);
   else
-    BG_HIP(hipMemcpyAsync(d, tmp, n * 32, hipMemcpyDeviceToDevice, s));
-  BG_HIP(hipGetLastError());
+    gather(tmp, d, log2, nullptr, nullptr, s);
 }
 
 void qap_h_run(uint32_t* a, uint32_t* b, uint32_t* c, uint32_t* h, uint32_t* tmp, int log2, bool canonical_out,
                hipStream_t s) {
+  (void)tmp;
   NttDomain& D = ntt_domain(log2);
   uint64_t n = D.n;
   uint32_t* v3[3] = {a, b, c};
   for (uint32_t* v : v3) {
-    // ifft (omega^-1 core) -> scale n^-1 g^j fused into the next bit reversal -> fft
-    k_bitrev<<<nblk(n, 256), 256, 0, s>>>(v, tmp, log2, nullptr);
-    core(tmp, log2, D.tw_inv.as<uint32_t>(), s);
-    k_bitrev<<<nblk(n, 256), 256, 0, s>>>(tmp, v, log2, D.sc_ninv_g.as<uint32_t>());
-    core(v, log2, D.tw_fwd.as<uint32_t>(), s);
+    // coefficients (bit-reversed, times n) -> n^-1 g^k scale on load -> coset evaluations
+    dif(v, v, log2, D.tw_inv.as<uint32_t>(), nullptr, s);
+    dit(v, v, log2, D.tw_fwd.as<uint32_t>(), D.sc_ninv_g_rev.as<uint32_t>(), s);
   }
   k_qap_point<<<nblk(n, 256), 256, 0, s>>>(a, b, c, n, to_fp(D.zinv));
-  k_bitrev<<<nblk(n, 256), 256, 0, s>>>(a, tmp, log2, nullptr);
-  core(tmp, log2, D.tw_inv.as<uint32_t>(), s);
-  k_scale<<<nblk(n, 256), 256, 0, s>>>(tmp, h, n,
-                                       canonical_out ? D.sc_ninv_gi_c.as<uint32_t>() : D.sc_ninv_gi.as<uint32_t>(),
-                                       Fp{});
   BG_HIP(hipGetLastError());
+  dif(a, a, log2, D.tw_inv.as<uint32_t>(), nullptr, s);
+  gather(a, h, log2, canonical_out ? D.sc_ninv_gi_c.as<uint32_t>() : D.sc_ninv_gi.as<uint32_t>(), nullptr, s);
 }
 
 }  // namespace bg

@@ -20,6 +20,7 @@ struct NttDomain {
   DevArray sc_ninv_g;    // n^-1 g^j (Montgomery)               -- ifft then coset_fft
   DevArray sc_ninv_gi;   // n^-1 g^-j (Montgomery)              -- coset_ifft output scale
   DevArray sc_ninv_gi_c; // n^-1 g^-j as raw integers: Montgomery product gives canonical out
+  DevArray sc_ninv_g_rev; // n^-1 g^rev(j) (Montgomery)          -- coset scale in bit-reversed order
   uint32_t ninv[8];      // n^-1 (Montgomery)
   uint32_t zinv[8];      // (g^n - 1)^-1 (Montgomery)
 };

@@ -72,6 +72,7 @@ struct bg_pk {
   bg::DevArray z, zc, ea, eb, ec, h, tmp, zraw;
   bg::MsmWork work[5];
   bg::HostBuf stage;
+  hipEvent_t ev[16] = {};  // phase marks of one proof (created once per key)
   uint64_t dev_bytes = 0;
 };
 
@@ -114,11 +115,11 @@ __global__ void k_from_mont(const uint32_t* __restrict__ in, uint32_t* __restric
 
 static unsigned nb(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
-// MSM schedule inside one proof (see bg_prove); BG16_PROVE_SCHEDULE=0|1 selects it.
+// Order of qap_h and the MSMs inside one proof (see bg_prove); BG16_PROVE_SCHEDULE=0..4.
 static int prove_schedule() {
   static int s = [] {
     const char* e = getenv("BG16_PROVE_SCHEDULE");
-    return e ? atoi(e) : 0;
+    return e ? atoi(e) : 3;
   }();
   return s;
 }
@@ -232,13 +233,9 @@ extern "C" int bg_pk_free(bg_pk* pk) {
   return guard([&] {
     if (!pk) return;
     std::lock_guard<std::mutex> lk(ctx().mu);
-    for (auto& w : pk->work) {
-      if (w.done) (void)hipEventDestroy(w.done);
-      for (DevBuf* b : {&w.keys, &w.vals, &w.ranks, &w.counts, &w.sorted, &w.buckets, &w.pkeys, &w.pflags, &w.ppts,
-                        &w.lvlS, &w.lvlT, &w.outPQ, &w.scan_tmp, &w.fix, &w.rfix})
-        b->release();
-      if (w.host_out.ptr) (void)hipHostFree(w.host_out.ptr);
-    }
+    for (auto& w : pk->work) w.release();
+    for (auto& e : pk->ev)
+      if (e) (void)hipEventDestroy(e);
     if (pk->stage.ptr) (void)hipHostFree(pk->stage.ptr);
     delete pk;
   });
@@ -276,8 +273,9 @@ extern "C" int bg_prove(bg_pk* pk, const uint32_t* witness, const uint32_t* rs, 
     // canonical scalars exist, so the latency-bound tails of one MSM overlap with the
     // accumulation of the others.
     hipStream_t s0 = c.stream, s1 = c.aux[0], s2 = c.aux[1], s3 = c.aux[2];
-    hipEvent_t ev[16];
-    for (auto& e : ev) BG_HIP(hipEventCreate(&e));
+    hipEvent_t* ev = pk->ev;
+    if (!ev[0])
+      for (int i = 0; i < 16; i++) BG_HIP(hipEventCreate(&ev[i]));
     BG_HIP(hipEventRecord(ev[0], s0));
     // 1. witness upload (16-bit words are packed on the device)
     uint32_t* z = pk->z.as<uint32_t>();
@@ -305,32 +303,55 @@ extern "C" int bg_prove(bg_pk* pk, const uint32_t* witness, const uint32_t* rs, 
       BG_HIP(hipGetLastError());
     }
     BG_HIP(hipEventRecord(ev[3], s0));
-    // 3. MSMs.  B2 (G2) starts on its own stream as soon as the canonical scalars exist.
-    //    The G1 MSMs run as one batch (one sort, accumulation and reduction over all their
-    //    windows): either A, B1, L right away on s1 with H alone after qap_h (schedule 0),
-    //    or A, B1, L, H together after qap_h on s0 (schedule 1).
+    // 3. qap_h and the MSMs.  The G1 MSMs other than H run as one batch (one sort,
+    //    accumulation and reduction over all their windows); H needs qap_h's output.
+    //    Schedules (BG16_PROVE_SCHEDULE; every one gives the same proof):
+    //   0: B2 on s3 and the G1 batch A, B1, L on s1 start at once; qap_h then H on s0
+    //   1: B2 on s3 at once; qap_h then the G1 batch A, B1, L, H on s0
+    //   2: as 0, but the G1 batch waits until B2's buckets are complete
+    //   3: as 0, but the G1 batch waits for qap_h (default: the shortest at 2^18)
+    //   4: qap_h first; then B2 on s3, the G1 batch A, B1, L on s1 and H on s0
     const int sched = prove_schedule();
     for (hipStream_t sx : {s1, s2, s3}) BG_HIP(hipStreamWaitEvent(sx, ev[2], 0));
-    BG_HIP(hipEventRecord(ev[9], s3));
-    msm_enqueue(2, pk->pb2.as<uint32_t>(), zc, nbq, 0, s3, pk->work[2]);
-    BG_HIP(hipEventRecord(ev[10], s3));
     const uint32_t* g1p[4] = {pk->pa.as<uint32_t>(), pk->pb1.as<uint32_t>(), pk->pl.as<uint32_t>(),
                               pk->ph.as<uint32_t>()};
     const uint32_t* g1s[4] = {zc, zc, zc + 8 * ni, pk->h.as<uint32_t>()};
     const uint64_t g1n[4] = {na, nbq, nl, nh};
-    if (sched == 0) {
-      BG_HIP(hipEventRecord(ev[5], s1));
-      msm_enqueue_batch(1, 3, g1p, g1s, g1n, 0, s1, pk->work[0]);
-      BG_HIP(hipEventRecord(ev[6], s1));
-    }
-    // 4. h (canonical, the MSM H scalars), then the H MSM (or the whole G1 batch)
-    {
+    auto run_qap = [&] {
       ScopedKTime kt("qap_h", s0);
       qap_h_run(pk->ea.as<uint32_t>(), pk->eb.as<uint32_t>(), pk->ec.as<uint32_t>(), pk->h.as<uint32_t>(),
                 pk->tmp.as<uint32_t>(), (int)lg, true, s0);
+      BG_HIP(hipEventRecord(ev[4], s0));
+    };
+    auto run_b2 = [&] {
+      BG_HIP(hipEventRecord(ev[9], s3));
+      msm_enqueue(2, pk->pb2.as<uint32_t>(), zc, nbq, 0, s3, pk->work[2]);
+      BG_HIP(hipEventRecord(ev[10], s3));
+    };
+    auto run_g1_batch3 = [&] {
+      BG_HIP(hipEventRecord(ev[5], s1));
+      msm_enqueue_batch(1, 3, g1p, g1s, g1n, 0, s1, pk->work[0]);
+      BG_HIP(hipEventRecord(ev[6], s1));
+    };
+    if (sched == 4) {
+      run_qap();
+      BG_HIP(hipStreamWaitEvent(s3, ev[4], 0));
+      BG_HIP(hipStreamWaitEvent(s1, ev[4], 0));
+      run_b2();
+      run_g1_batch3();
+    } else {
+      run_b2();
+      if (sched == 0 || sched == 2) {
+        if (sched == 2) BG_HIP(hipStreamWaitEvent(s1, pk->work[2].mid, 0));
+        run_g1_batch3();
+      }
+      run_qap();
+      if (sched == 3) {
+        BG_HIP(hipStreamWaitEvent(s1, ev[4], 0));
+        run_g1_batch3();
+      }
     }
-    BG_HIP(hipEventRecord(ev[4], s0));
-    if (sched == 0) {
+    if (sched != 1) {
       msm_enqueue(1, pk->ph.as<uint32_t>(), pk->h.as<uint32_t>(), nh, 0, s0, pk->work[1]);
     } else {
       BG_HIP(hipEventRecord(ev[5], s0));
@@ -359,24 +380,25 @@ extern "C" int bg_prove(bg_pk* pk, const uint32_t* witness, const uint32_t* rs, 
     bgh::P1 r_d1 = delta1.mul(rc), s_d1 = delta1.mul(sc), rs_d1 = delta1.mul(rsc);
     bgh::P2 s_d2 = delta2.mul(sc);
 
-    // 5. finish the MSMs and combine (groth16.bend:298-308); slots as prove.bend:64-116
+    // 5. finish the MSMs and combine (groth16.bend:298-308); slots as prove.bend:64-116.
+    //    Everything but the H term is combined while MSM H still runs on the device.
     uint32_t g1res[4 * 16], mB2[32], mHs[16];
     msm_finish(pk->work[0], g1res);
     const uint32_t *mA = g1res, *mB1 = g1res + 16, *mL = g1res + 32;
     const uint32_t* mH = g1res + 48;
     bgh::P1 A = bgh::load_affine<bgh::Fq>(mA).add(r_d1).add(alpha);
     bgh::P1 B1 = bgh::load_affine<bgh::Fq>(mB1).add(s_d1).add(beta1);
-    bgh::P1 sA_rB = A.mul(sc).add(B1.mul(rc));
+    bgh::P1 C0 = A.mul(sc).add(B1.mul(rc)).add(rs_d1.neg()).add(bgh::load_affine<bgh::Fq>(mL));
+    uint32_t proof[64];
+    bgh::store_affine(A, proof);
     msm_finish(pk->work[2], mB2);
     bgh::P2 B2 = bgh::load_affine<bgh::Fq2>(mB2).add(s_d2).add(beta2);
-    if (sched == 0) {
+    bgh::store_affine(B2, proof + 16);
+    if (sched != 1) {
       msm_finish(pk->work[1], mHs);
       mH = mHs;
     }
-    bgh::P1 C = sA_rB.add(rs_d1.neg()).add(bgh::load_affine<bgh::Fq>(mL)).add(bgh::load_affine<bgh::Fq>(mH));
-    uint32_t proof[64];
-    bgh::store_affine(A, proof);
-    bgh::store_affine(B2, proof + 16);
+    bgh::P1 C = C0.add(bgh::load_affine<bgh::Fq>(mH));
     bgh::store_affine(C, proof + 48);
     if (words16) {
       for (int i = 0; i < 64; i++) {
@@ -399,7 +421,6 @@ extern "C" int bg_prove(bg_pk* pk, const uint32_t* witness, const uint32_t* rs, 
       }
       t_ms[8] = std::chrono::duration<double, std::milli>(t1 - t0).count();
     }
-    for (auto& e : ev) (void)hipEventDestroy(e);
   });
 }
 

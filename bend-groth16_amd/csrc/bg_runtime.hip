@@ -1,6 +1,10 @@
 // Library context, error reporting and kernel timing (C ABI: bg_init, bg_last_error,
 // bg_set_timing, bg_timing_report, bg_timing_reset).
 #include "bg_runtime.hpp"
+#include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <thread>
 #include <stdarg.h>
 #include <string.h>
 #include <map>
@@ -78,6 +82,84 @@ static void timing_flush() {
     (void)hipEventDestroy(p.b);
   }
   g_pending.clear();
+}
+
+namespace {
+
+class HostPool {
+ public:
+  explicit HostPool(int workers) {
+    for (int i = 0; i < workers; i++) th_.emplace_back([this] { loop(); });
+  }
+  ~HostPool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : th_) t.join();
+  }
+  void run(int n, const std::function<void(int)>& f) {
+    std::lock_guard<std::mutex> serial(run_mu_);
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      job_ = &f;
+      n_ = n;
+      next_.store(0);
+      open_ = true;
+      gen_++;
+    }
+    cv_.notify_all();
+    work(f, n);
+    std::unique_lock<std::mutex> lk(mu_);
+    open_ = false;  // late wakers skip this job
+    done_cv_.wait(lk, [&] { return active_ == 0; });
+  }
+
+ private:
+  void work(const std::function<void(int)>& f, int n) {
+    for (int i; (i = next_.fetch_add(1)) < n;) f(i);
+  }
+  void loop() {
+    uint64_t seen = 0;
+    std::unique_lock<std::mutex> lk(mu_);
+    for (;;) {
+      cv_.wait(lk, [&] { return stop_ || (open_ && gen_ != seen); });
+      if (stop_) return;
+      seen = gen_;
+      const std::function<void(int)>* f = job_;
+      int n = n_;
+      active_++;
+      lk.unlock();
+      work(*f, n);
+      lk.lock();
+      if (--active_ == 0) done_cv_.notify_all();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex mu_, run_mu_;
+  std::condition_variable cv_, done_cv_;
+  const std::function<void(int)>* job_ = nullptr;
+  int n_ = 0, active_ = 0;
+  std::atomic<int> next_{0};
+  uint64_t gen_ = 0;
+  bool open_ = false, stop_ = false;
+};
+
+}  // namespace
+
+void host_parallel(int n, const std::function<void(int)>& f) {
+  if (n <= 0) return;
+  static HostPool* pool = [] {
+    unsigned hc = std::thread::hardware_concurrency();
+    int w = (int)std::min<unsigned>(hc > 1 ? hc - 1 : 0, 15);
+    return new HostPool(w);  // lives until the process ends
+  }();
+  if (n == 1) {
+    f(0);
+    return;
+  }
+  pool->run(n, f);
 }
 
 }  // namespace bg

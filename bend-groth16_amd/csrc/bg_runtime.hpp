@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 #include <mutex>
+#include <functional>
 #include "../../include/bg16.h"
 
 namespace bg {
@@ -133,7 +134,17 @@ struct HostBuf {
     }
     return ptr;
   }
+  void release() {
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+  }
 };
+
+// Persistent host worker pool for the small host-side reductions that finish each MSM
+// (thread start-up would otherwise dominate them).  run(n, f) calls f(0..n-1) on the
+// workers and the caller and returns when all calls are done; calls are serialised.
+void host_parallel(int n, const std::function<void(int)>& f);
 
 // Per-process device context.  The library drives one device; every entry point runs
 // on the context stream and synchronises before returning host-visible results.

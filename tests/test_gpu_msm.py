@@ -176,3 +176,46 @@ def test_msm_batch_closed_form_large(bg):
     got = bendg.msm_batch(1, pts_l, sc_l).view(np.uint64)
     for i in range(len(sizes)):
         assert np.array_equal(got[i], want[i])
+
+
+def _closed(group, pts, ks, sc, **kw):
+    want = _mul_gen(group, O.fr_dot(sc, ks))
+    return np.array_equal(_msm(group, pts, sc, **kw), want)
+
+
+@pytest.mark.parametrize("group,log2n,kind", [
+    (1, 16, "ones"), (1, 16, "equal"), (1, 16, "small"), (1, 17, "three"),
+    (2, 13, "ones"), (2, 13, "three")])
+def test_msm_skewed_scalars(bg, group, log2n, kind):
+    """Scalar distributions that concentrate digits in few buckets (witness values 0/1,
+    one repeated value, small integers, three distinct values): buckets spread over
+    thousands of accumulation slices, merged by the segmented tree levels."""
+    rng = np.random.default_rng(500 + log2n + group)
+    n = 1 << log2n
+    pts, ks = _bases(group, rng, n)
+    if kind == "ones":
+        sc = np.zeros((n, 4), dtype=np.uint64)
+        sc[:, 0] = 1
+        sc[::7, 0] = 0
+    elif kind == "equal":
+        sc = np.repeat(O.rand_scalars(rng, 1), n, axis=0)
+    elif kind == "small":
+        sc = np.zeros((n, 4), dtype=np.uint64)
+        sc[:, 0] = rng.integers(0, 16, size=n)
+    else:
+        vals = O.rand_scalars(rng, 3)
+        sc = vals[rng.integers(0, 3, size=n)]
+    print(f"msm skewed G{group} 2^{log2n} {kind}", flush=True)
+    assert _closed(group, pts, ks, sc)
+
+
+@pytest.mark.parametrize("group,log2n,c", [(1, 17, 12), (1, 17, 14), (2, 13, 12), (2, 13, 13)])
+def test_msm_short_top_window(bg, group, log2n, c):
+    """Window widths whose top window holds only a few digit values (254-bit scalars):
+    every scalar lands in one of a handful of top-window buckets."""
+    rng = np.random.default_rng(600 + c + group)
+    n = 1 << log2n
+    pts, ks = _bases(group, rng, n)
+    sc = O.rand_scalars(rng, n)
+    print(f"msm top window G{group} 2^{log2n} c={c}", flush=True)
+    assert _closed(group, pts, ks, sc, window_bits=c)
