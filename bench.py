@@ -9,6 +9,19 @@ full prove: evaluation vectors, qap_h (7 NTTs of 2^(K+1)), the five MSMs (A, B1,
 G1; B2 in G2) and the proof combination; the output is the affine proof (A, B, C).
 
 Prints one JSON line (rank 0).  With --gpus N > 1 it exits: this build measures one GPU.
+
+Other modes (one JSON line each):
+  --mode batch       64 SquareChain proofs, K uniform in 10..18 by seed, each with its own
+                     key and witness: proofs/s one proof at a time vs the proving engine
+                     (concurrent slots); plus 256 independent G1 MSMs with sizes
+                     log-uniform in 2^12..2^20: one call each vs one batched call.
+  --mode serve       Poisson stream of proof requests, K in {12,14,16,18,20} with weights
+                     0.3/0.3/0.2/0.15/0.05, all five keys resident, swept arrival rates:
+                     proofs/s and p50/p99 latency; every proof equals the same proof made
+                     alone.
+  --mode primitives  G1 MSM 2^20 and 2^22, G2 MSM 2^20 (bases k_i G, uniform scalars,
+                     checked against the closed form) and Fr NTT 2^20 / 2^24 forward and
+                     inverse (round trip checked).
 """
 from __future__ import annotations
 
@@ -44,11 +57,19 @@ class Hip:
         self.L.hipFree.argtypes = [ctypes.c_void_p]
         self.L.hipDeviceSynchronize.argtypes = []
 
-    def upload(self, arr: np.ndarray) -> int:
+    def alloc(self, nbytes: int) -> int:
         p = ctypes.c_void_p()
-        assert self.L.hipMalloc(ctypes.byref(p), arr.nbytes) == 0
-        assert self.L.hipMemcpy(p, arr.ctypes.data_as(ctypes.c_void_p), arr.nbytes, 1) == 0
+        assert self.L.hipMalloc(ctypes.byref(p), max(nbytes, 4)) == 0
         return p.value
+
+    def upload(self, arr: np.ndarray) -> int:
+        p = self.alloc(arr.nbytes)
+        assert self.L.hipMemcpy(ctypes.c_void_p(p), arr.ctypes.data_as(ctypes.c_void_p), arr.nbytes, 1) == 0
+        return p
+
+    def download(self, p: int, arr: np.ndarray) -> np.ndarray:
+        assert self.L.hipMemcpy(arr.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(p), arr.nbytes, 2) == 0
+        return arr
 
     def sync(self) -> None:
         assert self.L.hipDeviceSynchronize() == 0
@@ -129,6 +150,307 @@ def cpu_baseline(key: Path) -> dict:
                       "Pippenger c=8/10 unsigned windows, radix-2 NTT)"}
 
 
+# ------------------------------------------------------------------ engine / batch modes
+
+def engine_rate(pk, dz, rs, count: int, slots: int, ref) -> dict:
+    """count proofs of one key through the proving engine; proofs/s and bit-equality."""
+    eng = bendg.Engine(slots=slots)
+    for t in [eng.submit(pk, dz, rs) for _ in range(slots)]:  # slot work arrays
+        eng.wait(t)
+    t0 = time.perf_counter()
+    tickets = [eng.submit(pk, dz, rs) for _ in range(count)]
+    lat = []
+    for t in tickets:
+        p, tm = eng.wait(t)
+        lat.append(tm["latency"])
+        if not np.array_equal(p, ref):
+            raise SystemExit("engine proof differs from the single proof")
+    dt = time.perf_counter() - t0
+    eng.close()
+    return {"proofs": count, "slots": slots, "proofs_per_s": round(count / dt, 2)}
+
+
+def witness_files(d: Path):
+    z = bendg.pack16(np.fromfile(d / "witness.bin", dtype=np.uint32).reshape(-1, 16)).reshape(-1)
+    rs = bendg.pack16(np.fromfile(d / "rs.bin", dtype=np.uint32).reshape(-1, 16)).reshape(-1)
+    return z, rs
+
+
+def batch_mode(a) -> dict:
+    rng = np.random.default_rng(a.seed)
+    ks = [int(k) for k in rng.integers(10, 19, size=64)]
+    base = Path(a.keys) / "batch"
+    dirs = [ensure_key(k, 1000 + i, base) for i, k in enumerate(ks)]
+    bendg.init(0)
+    hip = Hip()
+    pks = [bendg.ProvingKey(d) for d in dirs]
+    wz = [witness_files(d) for d in dirs]
+    dzs = [hip.upload(z) for z, _ in wz]
+    log(f"64 keys loaded, K = {ks}")
+    # one proof at a time (the single-proof path), after one warm proof per key
+    ref = [pk.prove_device(dz, rs)[0] for pk, dz, (_, rs) in zip(pks, dzs, wz)]
+    hip.sync()
+    t0 = time.perf_counter()
+    for pk, dz, (_, rs), r in zip(pks, dzs, wz, ref):
+        p, _ = pk.prove_device(dz, rs)
+        if not np.array_equal(p, r):
+            raise SystemExit("single proof not reproducible")
+    single_s = time.perf_counter() - t0
+    # the proving engine: all 64 submitted at once, slots prove concurrently
+    res = {}
+    for slots in a.slots:
+        eng = bendg.Engine(slots=slots)
+        for t in [eng.submit(pk, dz, rs) for pk, dz, (_, rs) in zip(pks, dzs, wz)]:
+            eng.wait(t)  # warm: every slot's work arrays at their largest
+        t0 = time.perf_counter()
+        tickets = [eng.submit(pk, dz, rs) for pk, dz, (_, rs) in zip(pks, dzs, wz)]
+        lat = []
+        for t, r in zip(tickets, ref):
+            p, tm = eng.wait(t)
+            lat.append(tm["latency"])
+            if not np.array_equal(p, r):
+                raise SystemExit("batched proof differs from the single proof")
+        dt = time.perf_counter() - t0
+        eng.close()
+        res[slots] = {"proofs_per_s": round(64 / dt, 2), "wall_ms": round(dt * 1e3, 1),
+                      "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                      "p99_ms": round(float(np.percentile(lat, 99)), 2)}
+        log(f"engine slots={slots}: {res[slots]}")
+    best = max(res.values(), key=lambda v: v["proofs_per_s"])
+    single_rate = 64 / single_s
+    msm = batch_msm_256(a, hip)
+    for pk in pks:
+        pk.close()
+    return {"metric": "batched proofs/s, 64 SquareChain proofs K~U{10..18} (bit-exact)",
+            "value": best["proofs_per_s"], "unit": "proofs/s", "n_gpus": 1, "higher_is_better": True,
+            "single_proofs_per_s": round(single_rate, 2), "batched_vs_single": round(best["proofs_per_s"] / single_rate, 3),
+            "engine": {str(k): v for k, v in res.items()}, "ks": ks, "dtype": "u32 (exact BN254 integer arithmetic)",
+            "data": "synthetic (own trapdoor setup per proof, seeds 1000+i)", "msm_256": msm,
+            "config": {"workload": "64 proofs, K uniform in 10..18, own key and witness each, witnesses resident in HBM",
+                       "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES")}}
+
+
+def batch_msm_256(a, hip) -> dict:
+    """256 independent G1 MSMs, sizes 2^U(12,20): one bg_msm_g1 call each vs one
+    bg_msm_batch call (device-resident inputs); results must agree bit for bit."""
+    rng = np.random.default_rng(a.seed + 1)
+    ns = [int(round(2 ** x)) for x in rng.uniform(12, 20, size=256)]
+    pool_n = 1 << 20
+    pts = bendg.gen_bases(1, 77, pool_n)
+    dpts = hip.upload(pts)
+    tot = sum(ns)
+    sc = rng.integers(0, 2**32, size=(tot, 8), dtype=np.uint32)
+    sc[:, 7] &= np.uint32(0x1FFFFFFF)
+    dsc = hip.upload(sc)
+    offs = np.concatenate([[0], np.cumsum(ns)[:-1]]).astype(np.uint64)
+    L = bendg.lib()
+    flags = bendg.FMT_PACKED | bendg.MEM_DEVICE
+    dout = hip.alloc(256 * 64)
+    one = []
+
+    def seq():
+        for i, n in enumerate(ns):
+            bendg._check(L.bg_msm_g1(ctypes.c_void_p(dpts), ctypes.c_void_p(dsc + 32 * int(offs[i])), n,
+                                     ctypes.c_void_p(dout + 64 * i), flags, 0))
+
+    def bat():
+        pp = (ctypes.c_void_p * 256)(*([dpts] * 256))
+        sp = (ctypes.c_void_p * 256)(*[dsc + 32 * int(o) for o in offs])
+        nn = np.array(ns, dtype=np.uint64)
+        bendg._check(L.bg_msm_batch(1, 256, ctypes.cast(pp, ctypes.c_void_p), ctypes.cast(sp, ctypes.c_void_p),
+                                    bendg._p(nn), ctypes.c_void_p(dout), flags, 0))
+
+    seq()
+    hip.sync()
+    t0 = time.perf_counter()
+    seq()
+    hip.sync()
+    t_seq = time.perf_counter() - t0
+    r_seq = hip.download(dout, np.zeros((256, 16), np.uint32)).copy()
+    bat()
+    hip.sync()
+    t0 = time.perf_counter()
+    bat()
+    hip.sync()
+    t_bat = time.perf_counter() - t0
+    r_bat = hip.download(dout, np.zeros((256, 16), np.uint32))
+    if not np.array_equal(r_seq, r_bat):
+        raise SystemExit("batched MSM results differ from the one-at-a-time results")
+    for p in (dpts, dsc, dout):
+        hip.free(p)
+    log(f"256 MSMs ({tot} points): one at a time {t_seq * 1e3:.1f} ms, batched {t_bat * 1e3:.1f} ms")
+    return {"msms": 256, "points": tot, "sequential_ms": round(t_seq * 1e3, 2), "batched_ms": round(t_bat * 1e3, 2),
+            "batched_vs_sequential": round(t_seq / t_bat, 3), "points_per_s_batched": round(tot / t_bat, 0)}
+
+
+def serve_mode(a) -> dict:
+    import threading
+    kset = [12, 14, 16, 18, 20]
+    wts = [0.3, 0.3, 0.2, 0.15, 0.05]
+    dirs = {k: ensure_key(k, 42 + k, Path(a.keys)) for k in kset}
+    bendg.init(0)
+    hip = Hip()
+    pks = {k: bendg.ProvingKey(d) for k, d in dirs.items()}
+    wz = {k: witness_files(d) for k, d in dirs.items()}
+    dz = {k: hip.upload(wz[k][0]) for k in kset}
+    solo, alone = {}, {}
+    for k in kset:
+        pks[k].prove_device(dz[k], wz[k][1])
+        t0 = time.perf_counter()
+        solo[k], _ = pks[k].prove_device(dz[k], wz[k][1])
+        alone[str(k)] = round(1e3 * (time.perf_counter() - t0), 2)
+        log(f"K={k}: alone {alone[str(k)]} ms")
+    rates = []
+    for rate in a.rates:
+        eng = bendg.Engine(slots=a.slots[0])
+        for k in sorted(kset, reverse=True):  # warm every slot, largest size first
+            for t in [eng.submit(pks[k], dz[k], wz[k][1]) for _ in range(a.slots[0])]:
+                eng.wait(t)
+        rng = np.random.default_rng(a.seed + int(rate))
+        n = int(rate * a.duration)
+        gaps = rng.exponential(1.0 / rate, size=n)
+        kinds = rng.choice(kset, size=n, p=wts)
+        done = []
+        lock = threading.Lock()
+        q = []
+        cv = threading.Condition(lock)
+        finished = [False]
+
+        def collector():
+            while True:
+                with cv:
+                    while not q and not finished[0]:
+                        cv.wait()
+                    if not q:
+                        return
+                    t, k, t_arr = q.pop(0)
+                p, tm = eng.wait(t)
+                # latency = arrival to the engine's completion time (not to this wait,
+                # which runs in submission order)
+                done.append((k, tm["latency"] * 1e-3, tm, np.array_equal(p, solo[k])))
+
+        th = threading.Thread(target=collector)
+        th.start()
+        t_start = time.perf_counter()
+        t_next = t_start
+        for g, k in zip(gaps, kinds):
+            t_next += g
+            dt = t_next - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            t_arr = time.perf_counter()
+            t = eng.submit(pks[int(k)], dz[int(k)], wz[int(k)][1])  # submit time ~ arrival time
+            with cv:
+                q.append((t, int(k), t_arr))
+                cv.notify()
+        with cv:
+            finished[0] = True
+            cv.notify()
+        th.join()
+        wall = time.perf_counter() - t_start
+        eng.close()
+        lat = np.array([d[1] for d in done]) * 1e3
+        ok = all(d[3] for d in done)
+        if not ok:
+            raise SystemExit("a served proof differs from the same proof made alone")
+        per_k = {str(k): round(float(np.percentile([d[1] * 1e3 for d in done if d[0] == k], 99)), 2)
+                 for k in kset if any(d[0] == k for d in done)}
+        r = {"offered_per_s": rate, "proofs": len(done), "proofs_per_s": round(len(done) / wall, 2),
+             "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+             "p99_ms_by_k": per_k, "all_equal_alone": ok}
+        log(f"rate {rate}/s: {r}")
+        rates.append(r)
+    for pk in pks.values():
+        pk.close()
+    best = max(rates, key=lambda v: v["proofs_per_s"])
+    return {"metric": "served proofs/s and p99 latency, Poisson arrivals, K in {12..20} (bit-exact)",
+            "value": best["proofs_per_s"], "unit": "proofs/s", "n_gpus": 1, "higher_is_better": True,
+            "p99_ms_at_best": best["p99_ms"], "rates": rates, "alone_ms": alone, "slots": a.slots[0],
+            "duration_s_per_rate": a.duration, "dtype": "u32 (exact BN254 integer arithmetic)",
+            "data": "synthetic (own trapdoor setup per K, seed 42+K)",
+            "config": {"workload": "Poisson requests, K 12/14/16/18/20 w 0.3/0.3/0.2/0.15/0.05, keys resident",
+                       "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES")}}
+
+
+def primitives_mode(a) -> dict:
+    bendg.init(0)
+    hip = Hip()
+    L = bendg.lib()
+    out = {}
+    rng = np.random.default_rng(a.seed)
+    R = 21888242871839275222246405745257275088548364400416034343698204186575808495617
+    flags = bendg.FMT_PACKED | bendg.MEM_DEVICE
+    for group, lg in ((1, 20), (1, 22), (2, 20)):
+        n = 1 << lg
+        t = time.perf_counter()
+        pts, ks = bendg.gen_bases(group, 100 + lg, n, with_k=True)
+        log(f"G{group} 2^{lg}: bases in {time.perf_counter() - t:.1f} s")
+        sc = rng.integers(0, 2**32, size=(n, 8), dtype=np.uint32)
+        sc[:, 7] &= np.uint32(0x1FFFFFFF)
+        dp, ds = hip.upload(pts), hip.upload(sc)
+        pw = 16 if group == 1 else 32
+        dout = hip.alloc(pw * 4)
+        f = L.bg_msm_g1 if group == 1 else L.bg_msm_g2
+        call = lambda: bendg._check(f(ctypes.c_void_p(dp), ctypes.c_void_p(ds), n, ctypes.c_void_p(dout), flags, 0))  # noqa: E731
+        call()
+        hip.sync()
+        reps = 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            call()
+        hip.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        got = hip.download(dout, np.zeros(pw, np.uint32))
+        # closed form: (sum a_i k_i mod r) G
+        a64 = sc.view(np.uint64).reshape(-1, 4)
+        k64 = ks.view(np.uint64).reshape(-1, 4)
+        tot = 0
+        for i in range(0, n, 1 << 16):
+            av = [int(x[0]) | int(x[1]) << 64 | int(x[2]) << 128 | int(x[3]) << 192 for x in a64[i:i + (1 << 16)]]
+            kv = [int(x[0]) | int(x[1]) << 64 | int(x[2]) << 128 | int(x[3]) << 192 for x in k64[i:i + (1 << 16)]]
+            tot = (tot + sum(x * y for x, y in zip(av, kv))) % R
+        kt = np.array([(tot >> (32 * j)) & 0xFFFFFFFF for j in range(8)], dtype=np.uint32)
+        want = bendg.fixed_base_mul(group, kt).reshape(-1)
+        ok = bool(np.array_equal(got, want))
+        info = bendg.msm_plan_info(n, group)
+        out[f"msm_g{group}_2^{lg}"] = {"ms": round(ms, 3), "points_per_s": round(n / ms * 1e3, 0), "c": info["c"],
+                                       "closed_form_ok": ok}
+        log(f"  MSM G{group} 2^{lg}: {ms:.3f} ms, closed form {'ok' if ok else 'MISMATCH'}")
+        for p in (dp, ds, dout):
+            hip.free(p)
+        if not ok:
+            raise SystemExit("primitive MSM differs from its closed form")
+    for lg in (20, 24):
+        n = 1 << lg
+        x = rng.integers(0, 2**32, size=(n, 8), dtype=np.uint32)
+        x[:, 7] &= np.uint32(0x0FFFFFFF)
+        d = hip.upload(x)
+        res = {}
+        for name, dirn in (("fft", bendg.NTT_FFT), ("ifft", bendg.NTT_IFFT)):
+            bendg._check(L.bg_ntt(ctypes.c_void_p(d), lg, dirn, flags))
+            hip.sync()
+            reps = 5
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                bendg._check(L.bg_ntt(ctypes.c_void_p(d), lg, dirn, flags))
+            hip.sync()
+            res[name] = (time.perf_counter() - t0) * 1e3 / reps
+        # round trip: 6 ffts and 6 iffts above cancel exactly
+        back = hip.download(d, np.zeros_like(x))
+        ok = bool(np.array_equal(back, x))
+        hip.free(d)
+        byts = 64 * n
+        out[f"ntt_2^{lg}"] = {"fft_ms": round(res["fft"], 3), "ifft_ms": round(res["ifft"], 3),
+                              "fft_GBps_algorithmic": round(byts / res["fft"] / 1e6, 1), "round_trip_ok": ok}
+        log(f"  NTT 2^{lg}: fft {res['fft']:.3f} ms, ifft {res['ifft']:.3f} ms, round trip {'ok' if ok else 'BAD'}")
+        if not ok:
+            raise SystemExit("NTT round trip failed")
+    return {"metric": "primitive MSM / NTT times (bit-exact)", "value": out["msm_g1_2^20"]["ms"], "unit": "ms",
+            "n_gpus": 1, "higher_is_better": False, "primitives": out, "dtype": "u32 (exact BN254 integer arithmetic)",
+            "data": "synthetic (bases k_i G seeded, uniform scalars)",
+            "config": {"workload": "G1 MSM 2^20/2^22, G2 MSM 2^20, Fr NTT 2^20/2^24"}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,11 +460,28 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--keys", default=os.environ.get("BG16_KEYS", "/tmp/bg16_keys"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve", "primitives"])
+    ap.add_argument("--slots", type=lambda v: [int(x) for x in v.split(",")], default=[4, 8],
+                    help="engine slots (batch: comma list to sweep)")
+    ap.add_argument("--rates", type=lambda v: [float(x) for x in v.split(",")], default=[100.0, 200.0, 300.0],
+                    help="serve: arrival rates in proofs/s")
+    ap.add_argument("--duration", type=float, default=20.0, help="serve: seconds per arrival rate")
+    ap.add_argument("--hw-queues", type=int, default=0,
+                    help="HIP hardware queues for this process (each proof in flight uses four streams); "
+                         "0 keeps the environment's value for the single mode and uses 16 for the others")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
         return 2
+    # must precede the first HIP call of the process
+    hwq = a.hw_queues or (16 if a.mode != "single" else 0)
+    if hwq:
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, max(1, hwq)))
+    if a.mode != "single":
+        line = {"batch": batch_mode, "serve": serve_mode, "primitives": primitives_mode}[a.mode](a)
+        print(json.dumps(line), flush=True)
+        return 0
 
     key = ensure_key(a.k, a.seed, Path(a.keys))
     bendg.init(0)
@@ -195,6 +534,9 @@ def main() -> int:
     for k, v in sorted(alone.items()):
         log(f"  {k:22s} {v['avg_ms']:8.3f} ms/launch (n={v['n']})")
     roof = roofline(alone, peak)
+    log("engine (concurrent proofs of this key)")
+    batched = engine_rate(pk, dz, rs, 32, 4, ref)
+    batched["vs_single"] = round(batched["proofs_per_s"] / (1e3 / ms), 3)
     line = {
         "metric": METRIC, "value": round(ms, 3), "unit": "ms/proof", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(ms, 3), "higher_is_better": False, "scaling": "weak",
@@ -210,6 +552,7 @@ def main() -> int:
         "kernel_ms_per_proof_overlapped": per_proof,
         "kernel_ms_standalone": {k: v["avg_ms"] for k, v in alone.items()},
         "roofline": roof,
+        "batched_same_key": batched,
     }
     if not a.no_cpu_baseline:
         log("cpu baseline (C oracle, one proof)")

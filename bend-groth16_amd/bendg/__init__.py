@@ -76,6 +76,14 @@ def lib() -> ctypes.CDLL:
         "bg_probe_rate": [I, U64, P],
         "bg_msm_plan_info": [U64, I, I, P],
         "bg_msm_batch": [I, ctypes.c_uint32, P, P, P, P, I, I],
+        "bg_engine_create": [I, U64, ctypes.POINTER(ctypes.c_void_p)],
+        "bg_engine_submit": [P, P, P, P, I, ctypes.POINTER(ctypes.c_uint64)],
+        "bg_engine_wait": [P, U64, P, P],
+        "bg_engine_info": [P, P],
+        "bg_engine_destroy": [P],
+        "bg_prove_batch": [I, P, P, P, P, I, I],
+        "bg_fixed_base_mul": [I, P, U64, P, I],
+        "bg_gen_bases": [I, U64, U64, P, P, I],
     }
     for name, args in sigs.items():
         fn = getattr(L, name, None)
@@ -250,6 +258,22 @@ def qap_h(a: np.ndarray, b: np.ndarray, c: np.ndarray, log2n: int, fmt: str = "p
     return h
 
 
+def fixed_base_mul(group: int, scalars: np.ndarray, threads: int = 0) -> np.ndarray:
+    """k_i * G for canonical packed scalars (n, 8) u32 -> (n, 16 | 32) packed affine."""
+    s = _u32(scalars).reshape(-1, 8)
+    out = np.zeros((s.shape[0], 16 if group == 1 else 32), dtype=np.uint32)
+    _check(lib().bg_fixed_base_mul(group, _p(s), s.shape[0], _p(out), threads))
+    return out
+
+
+def gen_bases(group: int, seed: int, n: int, with_k: bool = False, threads: int = 0):
+    """n seeded synthetic bases k_i G (packed affine); with_k also returns the k_i."""
+    out = np.zeros((n, 16 if group == 1 else 32), dtype=np.uint32)
+    k = np.zeros((n, 8), dtype=np.uint32) if with_k else None
+    _check(lib().bg_gen_bases(group, seed, n, _p(out), _p(k), threads))
+    return (out, k) if with_k else out
+
+
 def msm_plan_info(n: int, group: int = 1, window_bits: int = 0) -> dict:
     info = np.zeros(8, dtype=np.uint64)
     _check(lib().bg_msm_plan_info(n, group, window_bits, _p(info)))
@@ -281,4 +305,4 @@ def timing_report() -> dict[str, tuple[float, int]]:
     return out
 
 
-from .prover import ProvingKey, setup_square_chain, format_proof  # noqa: E402,F401
+from .prover import Engine, ProvingKey, prove_batch, setup_square_chain, format_proof  # noqa: E402,F401

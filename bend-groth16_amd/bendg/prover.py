@@ -74,6 +74,75 @@ class ProvingKey:
             pass
 
 
+class Engine:
+    """Continuous-batching prover (``bg_engine_*``): ``submit`` queues a proof and returns
+    a ticket, ``wait`` returns (proof, {queued, proving, latency} ms).  Each proof equals
+    what ``ProvingKey.prove`` returns for the same inputs."""
+
+    def __init__(self, slots: int = 4, hbm_budget: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib().bg_engine_create(slots, hbm_budget, ctypes.byref(h)))
+        self._h = h
+        self._keep = {}
+
+    def submit(self, pk: ProvingKey, witness, rs, fmt: str = "words16") -> int:
+        """witness / rs as ``ProvingKey.prove``; or witness = device pointer (int,
+        packed, kept valid by the caller until ``wait``)."""
+        t = ctypes.c_uint64()
+        r = np.ascontiguousarray(rs, dtype=np.uint32).reshape(-1)
+        if isinstance(witness, int):
+            if r.size != 16:
+                raise ValueError("rs must hold two packed Fr elements")
+            _check(lib().bg_engine_submit(self._h, pk.handle, ctypes.c_void_p(witness), _p(r), FMT_PACKED | 2,
+                                          ctypes.byref(t)))
+        else:
+            ew = 16 if fmt == "words16" else 8
+            w = np.ascontiguousarray(witness, dtype=np.uint32).reshape(-1)
+            if r.size != 2 * ew or w.size != (pk.header[1] + pk.header[2]) * ew:
+                raise ValueError("witness / rs sizes do not match the key")
+            flag = FMT_WORDS16 if fmt == "words16" else FMT_PACKED
+            _check(lib().bg_engine_submit(self._h, pk.handle, _p(w), _p(r), flag, ctypes.byref(t)))
+        self._keep[t.value] = 16 if (fmt == "words16" and not isinstance(witness, int)) else 8
+        return t.value
+
+    def wait(self, ticket: int):
+        ew = self._keep.pop(ticket, 8)
+        out = np.zeros(8 * ew, dtype=np.uint32)
+        tm = np.zeros(3, dtype=np.float64)
+        _check(lib().bg_engine_wait(self._h, ticket, _p(out), _p(tm)))
+        return out, {"queued": float(tm[0]), "proving": float(tm[1]), "latency": float(tm[2])}
+
+    def info(self) -> dict:
+        v = np.zeros(4, dtype=np.uint64)
+        _check(lib().bg_engine_info(self._h, _p(v)))
+        return {"slots": int(v[0]), "reserved_bytes": int(v[1]), "budget_bytes": int(v[2]), "completed": int(v[3])}
+
+    def close(self):
+        if self._h:
+            lib().bg_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def prove_batch(pks, witnesses, rss, fmt: str = "words16", slots: int = 4) -> list:
+    """``bg_prove_batch``: one proof per (key, witness, rs), returned in order."""
+    n = len(pks)
+    ew = 16 if fmt == "words16" else 8
+    ws = [np.ascontiguousarray(w, dtype=np.uint32).reshape(-1) for w in witnesses]
+    rr = [np.ascontiguousarray(r, dtype=np.uint32).reshape(-1) for r in rss]
+    outs = [np.zeros(8 * ew, dtype=np.uint32) for _ in range(n)]
+    arr = lambda xs: ctypes.cast((ctypes.c_void_p * n)(*[x.ctypes.data for x in xs]), ctypes.c_void_p)  # noqa: E731
+    kp = ctypes.cast((ctypes.c_void_p * n)(*[pk.handle.value for pk in pks]), ctypes.c_void_p)
+    flag = FMT_WORDS16 if fmt == "words16" else FMT_PACKED
+    _check(lib().bg_prove_batch(n, kp, arr(ws), arr(rr), arr(outs), flag, slots))
+    return outs
+
+
 def setup_square_chain(directory: str | Path, log2: int, seed: int = 42, threads: int = 0) -> Path:
     """Write a SquareChain(2^log2) key directory (FORMAT.md files + trapdoor.bin)."""
     d = Path(directory)

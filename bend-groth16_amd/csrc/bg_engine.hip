@@ -1,0 +1,285 @@
+// Proving engine: continuous batching of Groth16 proofs on one device (C ABI:
+// bg_engine_create, bg_engine_submit, bg_engine_wait, bg_engine_info, bg_engine_destroy,
+// bg_prove_batch).
+//
+// Requests (key, witness, r/s) queue in arrival order.  Each worker thread owns one
+// ProveSlot (four streams and the work arrays of one proof) and takes the oldest request
+// whose work arrays fit the device-memory budget, so proofs of different sizes run
+// concurrently: one proof's latency-bound reductions and host finishing overlap the
+// others' accumulation.  Keys stay resident (bg_pk_load) and are shared read-only by all
+// slots.  Every proof is computed exactly as bg_prove computes it alone.
+//
+// The reference proves one instance per process run (bend2/prove.bend:129-162); this is
+// the serving layer the batched and served configurations run on.
+#include "bg_prover.hpp"
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <string.h>
+#include <thread>
+#include <vector>
+
+struct bg_engine;
+
+namespace bg {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+  uint64_t ticket = 0;
+  bg_pk* pk = nullptr;
+  std::vector<uint32_t> witness;  // host copy (empty when the witness is on the device)
+  const uint32_t* dev_witness = nullptr;
+  uint32_t rs[32];
+  int flags = 0;
+  uint64_t need = 0;  // device bytes of its work arrays
+  Clock::time_point t_submit, t_start, t_end;
+  uint32_t proof[128];
+  int status = 1;  // 1 pending, 0 ok, < 0 error
+  std::string err;
+};
+
+}  // namespace bg
+
+struct bg_engine {
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<bg::Request*> queue;
+  std::map<uint64_t, std::unique_ptr<bg::Request>> all;
+  std::vector<std::thread> workers;
+  std::vector<uint64_t> reserved;  // per slot: device bytes its work arrays may hold
+  uint64_t budget = 0, next_ticket = 1, completed = 0;
+  bool stop = false;
+  int device = 0;
+};
+
+namespace bg {
+
+static uint64_t total_reserved(const bg_engine* e) {
+  uint64_t t = 0;
+  for (uint64_t r : e->reserved) t += r;
+  return t;
+}
+
+static void worker_main(bg_engine* e, int slot_id) {
+  std::unique_ptr<ProveSlot> slot;
+  try {
+    slot = std::make_unique<ProveSlot>();
+  } catch (const Error&) {
+  }
+  for (;;) {
+    Request* r = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(e->mu);
+      for (;;) {
+        if (e->stop) return;
+        if (!e->queue.empty()) {
+          Request* head = e->queue.front();
+          uint64_t mine = e->reserved[slot_id];
+          uint64_t others = total_reserved(e) - mine;
+          uint64_t grown = std::max(mine, head->need);
+          if (others + grown <= e->budget) {
+            e->queue.pop_front();
+            e->reserved[slot_id] = grown;
+            r = head;
+            break;
+          }
+          if (mine && others + head->need <= e->budget) {
+            // the head fits once this idle slot gives its arrays back
+            e->queue.pop_front();
+            e->reserved[slot_id] = head->need;
+            r = head;
+            lk.unlock();
+            if (slot) slot->release_buffers();
+            lk.lock();
+            break;
+          }
+          if (mine) {
+            // idle and holding memory that a waiting request needs: give it back
+            e->reserved[slot_id] = 0;
+            lk.unlock();
+            if (slot) slot->release_buffers();
+            lk.lock();
+            e->cv_work.notify_all();
+            continue;
+          }
+        }
+        e->cv_work.wait(lk);
+      }
+    }
+    r->t_start = Clock::now();
+    int rc = 0;
+    std::string msg;
+    if (!slot) {
+      rc = BG_ERR_DEVICE;
+      msg = "engine worker could not create its streams";
+    } else {
+      rc = guard([&] {
+        prove_on(r->pk, *slot, r->dev_witness ? r->dev_witness : r->witness.data(), r->rs, r->proof, r->flags,
+                 nullptr);
+      });
+      if (rc < 0) msg = bg_last_error();
+    }
+    r->t_end = Clock::now();
+    {
+      std::lock_guard<std::mutex> lk(e->mu);
+      r->status = rc < 0 ? rc : 0;
+      r->err = msg;
+      e->completed++;
+      // actual footprint after the run (grow-only buffers)
+      if (slot) e->reserved[slot_id] = std::max(e->reserved[slot_id], slot->device_bytes());
+    }
+    e->cv_done.notify_all();
+    e->cv_work.notify_all();
+  }
+}
+
+}  // namespace bg
+
+using namespace bg;
+
+extern "C" int bg_engine_create(int slots, uint64_t hbm_budget_bytes, bg_engine** out) {
+  return guard([&] {
+    BG_REQUIRE(out && slots >= 1 && slots <= 64, BG_ERR_ARG, "bg_engine_create: slots %d out of 1..64", slots);
+    *out = nullptr;
+    Ctx& c = ctx();
+    auto e = std::make_unique<bg_engine>();
+    e->device = c.device;
+    if (hbm_budget_bytes == 0) {
+      size_t fr = 0, tot = 0;
+      BG_HIP(hipMemGetInfo(&fr, &tot));
+      hbm_budget_bytes = (uint64_t)(fr * 0.9);  // default: 90 % of the memory free now
+    }
+    e->budget = hbm_budget_bytes;
+    e->reserved.assign(slots, 0);
+    bg_engine* raw = e.get();
+    for (int i = 0; i < slots; i++) e->workers.emplace_back([raw, i] { worker_main(raw, i); });
+    *out = e.release();
+  });
+}
+
+extern "C" int bg_engine_submit(bg_engine* e, bg_pk* pk, const uint32_t* witness, const uint32_t* rs, int flags,
+                                uint64_t* ticket) {
+  return guard([&] {
+    BG_REQUIRE(e && pk && witness && rs && ticket, BG_ERR_ARG, "bg_engine_submit: null argument");
+    bool words16 = (flags & 1) == BG_FMT_WORDS16;
+    bool dev = (flags & BG_MEM_DEVICE) != 0;
+    BG_REQUIRE(!(dev && words16), BG_ERR_ARG, "bg_engine_submit: a device witness must be packed");
+    uint32_t hdr[8];
+    BG_REQUIRE(bg_pk_header(pk, hdr) == 0, BG_ERR_ARG, "bg_engine_submit: bad key");
+    auto r = std::make_unique<Request>();
+    r->pk = pk;
+    r->flags = flags;
+    const uint64_t nz = (uint64_t)hdr[1] + hdr[2];
+    const size_t ew = words16 ? 16 : 8;
+    if (dev)
+      r->dev_witness = witness;
+    else
+      r->witness.assign(witness, witness + nz * ew);
+    memcpy(r->rs, rs, 2 * ew * 4);
+    r->need = prove_bytes_estimate(pk);
+    std::lock_guard<std::mutex> lk(e->mu);
+    BG_REQUIRE(r->need <= e->budget, BG_ERR_NOMEM,
+               "bg_engine_submit: one proof of this key needs %llu bytes, budget is %llu",
+               (unsigned long long)r->need, (unsigned long long)e->budget);
+    r->ticket = e->next_ticket++;
+    r->t_submit = Clock::now();
+    *ticket = r->ticket;
+    e->queue.push_back(r.get());
+    e->all[r->ticket] = std::move(r);
+    e->cv_work.notify_all();
+  });
+}
+
+extern "C" int bg_engine_wait(bg_engine* e, uint64_t ticket, uint32_t* proof_out, double* t_ms) {
+  std::unique_ptr<Request> r;
+  int rc = guard([&] {
+    BG_REQUIRE(e && proof_out, BG_ERR_ARG, "bg_engine_wait: null argument");
+    std::unique_lock<std::mutex> lk(e->mu);
+    auto it = e->all.find(ticket);
+    BG_REQUIRE(it != e->all.end(), BG_ERR_ARG, "bg_engine_wait: unknown ticket %llu", (unsigned long long)ticket);
+    Request* q = it->second.get();
+    e->cv_done.wait(lk, [&] { return q->status != 1; });
+    r = std::move(it->second);
+    e->all.erase(it);
+  });
+  if (rc < 0) return rc;
+  if (r->status < 0) {
+    set_error("%s", r->err.c_str());
+    return r->status;
+  }
+  const size_t words = (r->flags & 1) == BG_FMT_WORDS16 ? 128 : 64;
+  memcpy(proof_out, r->proof, words * 4);
+  if (t_ms) {
+    using ms = std::chrono::duration<double, std::milli>;
+    t_ms[0] = ms(r->t_start - r->t_submit).count();  // queued
+    t_ms[1] = ms(r->t_end - r->t_start).count();     // proving
+    t_ms[2] = ms(r->t_end - r->t_submit).count();    // latency
+  }
+  return 0;
+}
+
+extern "C" int bg_engine_info(bg_engine* e, uint64_t* info4) {
+  return guard([&] {
+    BG_REQUIRE(e && info4, BG_ERR_ARG, "bg_engine_info: null argument");
+    std::lock_guard<std::mutex> lk(e->mu);
+    info4[0] = e->workers.size();
+    info4[1] = total_reserved(e);
+    info4[2] = e->budget;
+    info4[3] = e->completed;
+  });
+}
+
+extern "C" int bg_engine_destroy(bg_engine* e) {
+  return guard([&] {
+    if (!e) return;
+    {
+      std::unique_lock<std::mutex> lk(e->mu);
+      // finish what is queued or running, then stop the workers
+      e->cv_done.wait(lk, [&] {
+        for (auto& kv : e->all)
+          if (kv.second->status == 1) return false;
+        return true;
+      });
+      e->stop = true;
+    }
+    e->cv_work.notify_all();
+    for (auto& t : e->workers) t.join();
+    delete e;
+  });
+}
+
+extern "C" int bg_prove_batch(int count, bg_pk* const* pks, const uint32_t* const* witnesses,
+                              const uint32_t* const* rs, uint32_t* const* proofs, int flags, int slots) {
+  bg_engine* e = nullptr;
+  int rc = guard([&] {
+    BG_REQUIRE(count >= 0 && (count == 0 || (pks && witnesses && rs && proofs)), BG_ERR_ARG,
+               "bg_prove_batch: bad argument");
+  });
+  if (rc < 0 || count == 0) return rc;
+  rc = bg_engine_create(slots > 0 ? slots : 4, 0, &e);
+  if (rc < 0) return rc;
+  std::vector<uint64_t> tickets(count, 0);
+  int first_err = 0;
+  std::string msg;
+  for (int i = 0; i < count && !first_err; i++) {
+    int r = bg_engine_submit(e, pks[i], witnesses[i], rs[i], flags, &tickets[i]);
+    if (r < 0) {
+      first_err = r;
+      msg = bg_last_error();
+    }
+  }
+  for (int i = 0; i < count; i++) {
+    if (!tickets[i]) continue;
+    int r = bg_engine_wait(e, tickets[i], proofs[i], nullptr);
+    if (r < 0 && !first_err) {
+      first_err = r;
+      msg = bg_last_error();
+    }
+  }
+  bg_engine_destroy(e);
+  if (first_err) set_error("%s", msg.c_str());
+  return first_err;
+}

@@ -3,6 +3,8 @@
 #include "bg_curve.hpp"
 #include "bg_host_ec.hpp"
 #include <algorithm>
+#include <map>
+#include <memory>
 #include <thread>
 #include <vector>
 
@@ -61,6 +63,21 @@ MsmPlan msm_plan_batch(const uint64_t* ns, uint32_t M, int group, int window_bit
 }
 
 MsmPlan msm_plan(uint64_t n, int group, int window_bits) { return msm_plan_batch(&n, 1, group, window_bits); }
+
+uint64_t msm_work_bytes(const MsmPlan& p, int group) {
+  const uint64_t xb = group == 1 ? 128 : 256;  // XYZZ point bytes
+  const uint64_t ngroups = (uint64_t)p.M * p.W * p.G, nout = (uint64_t)p.M * p.W * p.H;
+  const uint64_t mcap = (p.acc_threads + 127) / 128;
+  uint64_t b = p.entries * 20                    // keys, vals, ranks, sorted pairs
+               + 8 * ((uint64_t)p.nb + 1)        // counts | offsets
+               + 4 * (1 + 2ull * p.nb)           // fix-up flags and list
+               + 4 * (2 + ngroups + ngroups / p.K)  // reduction exception lists
+               + (uint64_t)p.nb * xb             // buckets
+               + 2ull * p.acc_threads * (8 + xb)  // partial slots
+               + 4 * mcap * (8 + xb)             // merge levels
+               + 2 * ngroups * xb + 2 * nout * xb + (1u << 16);
+  return b + b / 8;  // grow-only buffers keep 1/8 slack
+}
 
 // ------------------------------------------------------------------ kernels
 
@@ -479,14 +496,59 @@ extern "C" int bg_msm_batch(int group, uint32_t count, const uint32_t* const* po
         off += n[i];
       }
     }
-    static MsmWork work[2];
-    MsmWork& w = work[group - 1];
+    // Each MSM's own planned window width; MSMs sharing a width run as one batched
+    // launch sequence (sizes 2^12 .. 2^20 in one call span four or five widths), the
+    // width classes on separate streams.
+    std::map<int, std::vector<uint32_t>> classes;
+    for (uint32_t i = 0; i < count; i++) {
+      int cw = window_bits > 0 ? window_bits : (n[i] ? msm_plan(n[i], group, 0).c : 0);
+      classes[cw].push_back(i);
+    }
+    static std::map<std::pair<int, int>, std::unique_ptr<MsmWork>> works;
+    hipStream_t streams[4] = {c.stream, c.aux[0], c.aux[1], c.aux[2]};
+    struct Job {
+      MsmWork* w;
+      std::vector<uint32_t> idx;
+    };
+    std::vector<Job> jobs;
+    std::vector<uint32_t> res(pw * count, 0);
     {
       ScopedKTime kt(group == 1 ? "msm_g1_batch" : "msm_g2_batch", c.stream);
-      msm_enqueue_batch(group, count, dp.data(), ds.data(), n.data(), window_bits, c.stream, w);
+      hipEvent_t start;
+      BG_HIP(hipEventCreateWithFlags(&start, hipEventDisableTiming));
+      BG_HIP(hipEventRecord(start, c.stream));  // host-to-device copies above are on c.stream
+      int k = 0;
+      for (auto& cl : classes) {
+        if (cl.first == 0) continue;  // empty MSMs: result stays infinity
+        auto& slot = works[{group, cl.first}];
+        if (!slot) slot = std::make_unique<MsmWork>();
+        std::vector<const uint32_t*> cp, cs;
+        std::vector<uint64_t> cn;
+        for (uint32_t i : cl.second) {
+          cp.push_back(dp[i]);
+          cs.push_back(ds[i]);
+          cn.push_back(n[i]);
+        }
+        hipStream_t s = streams[k++ % 4];
+        BG_HIP(hipStreamWaitEvent(s, start, 0));
+        msm_enqueue_batch(group, (uint32_t)cp.size(), cp.data(), cs.data(), cn.data(), cl.first, s, *slot);
+        jobs.push_back({slot.get(), cl.second});
+      }
+      for (auto& j : jobs) {
+        std::vector<uint32_t> r(pw * j.idx.size());
+        msm_finish(*j.w, r.data());
+        for (size_t q = 0; q < j.idx.size(); q++) memcpy(&res[pw * j.idx[q]], &r[pw * q], pw * 4);
+      }
+      for (hipStream_t s : streams) {
+        if (s == c.stream) continue;
+        hipEvent_t e;
+        BG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        BG_HIP(hipEventRecord(e, s));
+        BG_HIP(hipStreamWaitEvent(c.stream, e, 0));
+        (void)hipEventDestroy(e);
+      }
+      (void)hipEventDestroy(start);
     }
-    std::vector<uint32_t> res(pw * count);
-    msm_finish(w, res.data());
     if (dev)
       BG_HIP(hipMemcpy(outs, res.data(), res.size() * 4, hipMemcpyHostToDevice));
     else

@@ -46,6 +46,13 @@ struct MsmWork {
   int group = 1;
   hipEvent_t done = nullptr;
   hipEvent_t mid = nullptr;  // recorded once the buckets are complete (before the reductions)
+  uint64_t bytes() const {
+    uint64_t t = 0;
+    for (const DevBuf* b : {&keys, &vals, &ranks, &counts, &sorted, &buckets, &pkeys, &pflags, &ppts, &mkeys, &mflags,
+                            &mpts, &lvlS, &lvlT, &outPQ, &scan_tmp, &fix, &rfix, &ptrs})
+      t += b->cap;
+    return t;
+  }
   void release() {
     for (DevBuf* b : {&keys, &vals, &ranks, &counts, &sorted, &buckets, &pkeys, &pflags, &ppts, &mkeys, &mflags,
                       &mpts, &lvlS, &lvlT, &outPQ, &scan_tmp, &fix, &rfix, &ptrs})
@@ -57,6 +64,9 @@ struct MsmWork {
     done = mid = nullptr;
   }
 };
+
+// Device bytes the work buffers of a batch with this plan occupy (upper estimate).
+uint64_t msm_work_bytes(const MsmPlan& p, int group);
 
 // Enqueue the device part of an MSM on stream s: points (packed affine, device),
 // scalars (packed canonical, device), n.  Results land in w.host_out (pinned) when the

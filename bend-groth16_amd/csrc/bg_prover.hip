@@ -9,6 +9,7 @@
 //   (prove.bend:64-116) -> proof combination on the host (groth16.bend:298-308).
 #include "bg_runtime.hpp"
 #include "bg_msm.hpp"
+#include "bg_prover.hpp"
 #include "bg_ntt.hpp"
 #include "bg_field.hpp"
 #include "bg_host_ec.hpp"
@@ -68,12 +69,10 @@ struct bg_pk {
   bg::DevArray pa, pb1, pb2, ph, pl;
   uint32_t g1[3 * 16];  // alpha, beta, delta (packed affine)
   uint32_t g2[2 * 32];  // beta2, delta2
-  // per-proof device work space
-  bg::DevArray z, zc, ea, eb, ec, h, tmp, zraw;
-  bg::MsmWork work[5];
-  bg::HostBuf stage;
-  hipEvent_t ev[16] = {};  // phase marks of one proof (created once per key)
-  uint64_t dev_bytes = 0;
+  uint64_t dev_bytes = 0;  // resident key bytes
+  // slot used by bg_prove (created on first use; one bg_prove call per key at a time)
+  std::unique_ptr<bg::ProveSlot> own;
+  std::mutex own_mu;
 };
 
 namespace bg {
@@ -213,16 +212,6 @@ extern "C" int bg_pk_load(const char* dir, bg_pk** out) {
       upload_array(pr.second->coef, pr.first->coef.data(), 32ull * pr.first->nnz, total);
     }
     for (auto& q : qs) upload_array(*q.dst, q.host.data(), q.host.size() * 4, total);
-    // per-proof work arrays
-    pk->z.alloc(32 * nz);
-    pk->zc.alloc(32 * nz);
-    pk->zraw.alloc(64 * nz);
-    pk->ea.alloc(32 * dsz);
-    pk->eb.alloc(32 * dsz);
-    pk->ec.alloc(32 * dsz);
-    pk->h.alloc(32 * dsz);
-    pk->tmp.alloc(32 * dsz);
-    total += 32 * nz * 2 + 64 * nz + 32 * dsz * 5;
     pk->dev_bytes = total;
     (void)ntt_domain((int)lg);  // twiddle / scale tables resident with the key
     *out = pk.release();
@@ -232,11 +221,10 @@ extern "C" int bg_pk_load(const char* dir, bg_pk** out) {
 extern "C" int bg_pk_free(bg_pk* pk) {
   return guard([&] {
     if (!pk) return;
-    std::lock_guard<std::mutex> lk(ctx().mu);
-    for (auto& w : pk->work) w.release();
-    for (auto& e : pk->ev)
-      if (e) (void)hipEventDestroy(e);
-    if (pk->stage.ptr) (void)hipHostFree(pk->stage.ptr);
+    {
+      std::lock_guard<std::mutex> lk(pk->own_mu);
+      pk->own.reset();
+    }
     delete pk;
   });
 }
@@ -255,15 +243,81 @@ extern "C" int bg_pk_device_bytes(const bg_pk* pk, uint64_t* bytes) {
   });
 }
 
+namespace bg {
+
+ProveSlot::ProveSlot() {
+  (void)ctx();  // device current on this thread
+  for (auto& x : s) BG_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+  for (auto& e : ev) BG_HIP(hipEventCreate(&e));
+}
+
+ProveSlot::~ProveSlot() {
+  release_buffers();
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& x : s)
+    if (x) (void)hipStreamDestroy(x);
+}
+
+void ProveSlot::ensure(uint64_t nz, uint64_t d) {
+  if (nz > cap_nz) {
+    z.alloc(32 * nz);
+    zc.alloc(32 * nz);
+    zraw.alloc(64 * nz);
+    cap_nz = nz;
+  }
+  if (d > cap_d) {
+    for (DevArray* a : {&ea, &eb, &ec, &h, &tmp}) a->alloc(32 * d);
+    cap_d = d;
+  }
+}
+
+void ProveSlot::release_buffers() {
+  for (DevArray* a : {&z, &zc, &zraw, &ea, &eb, &ec, &h, &tmp}) a->release();
+  cap_nz = cap_d = 0;
+  for (auto& w : work) w.release();
+}
+
+uint64_t ProveSlot::device_bytes() const {
+  uint64_t t = 128 * cap_nz + 160 * cap_d;
+  for (auto& w : work) t += w.bytes();
+  return t;
+}
+
+uint64_t prove_bytes_estimate(const bg_pk* pk) {
+  const uint32_t ni = pk->hdr[1], nw = pk->hdr[2], lg = pk->hdr[3];
+  const uint64_t na = pk->hdr[4], nbq = pk->hdr[5], nh = pk->hdr[6], nl = pk->hdr[7];
+  const uint64_t nz = (uint64_t)ni + nw, d = 1ull << lg;
+  uint64_t g1n[3] = {na, nbq, nl};
+  uint64_t t = 128 * nz + 160 * d;
+  t += msm_work_bytes(msm_plan_batch(g1n, 3, 1, 0), 1);
+  t += msm_work_bytes(msm_plan(nh, 1, 0), 1);
+  t += msm_work_bytes(msm_plan(nbq, 2, 0), 2);
+  return t;
+}
+
+}  // namespace bg
+
 extern "C" int bg_prove(bg_pk* pk, const uint32_t* witness, const uint32_t* rs, uint32_t* proof_out, int flags,
                         double* t_ms) {
   return guard([&] {
     BG_REQUIRE(pk && witness && rs && proof_out, BG_ERR_ARG, "bg_prove: null argument");
+    std::lock_guard<std::mutex> lk(pk->own_mu);
+    if (!pk->own) pk->own = std::make_unique<ProveSlot>();
+    prove_on(pk, *pk->own, witness, rs, proof_out, flags, t_ms);
+  });
+}
+
+namespace bg {
+
+void prove_on(bg_pk* pk, ProveSlot& sl, const uint32_t* witness, const uint32_t* rs, uint32_t* proof_out, int flags,
+              double* t_ms) {
+  {
+    BG_REQUIRE(pk && witness && rs && proof_out, BG_ERR_ARG, "bg_prove: null argument");
     bool words16 = (flags & 1) == BG_FMT_WORDS16;
     bool dev_witness = (flags & BG_MEM_DEVICE) != 0;
     BG_REQUIRE(!(dev_witness && words16), BG_ERR_ARG, "bg_prove: a device witness must be packed");
-    Ctx& c = ctx();
-    std::lock_guard<std::mutex> lk(c.mu);
+    (void)ctx();  // device current on this thread
     auto t0 = std::chrono::steady_clock::now();
     const uint32_t nc = pk->hdr[0], ni = pk->hdr[1], nw = pk->hdr[2], lg = pk->hdr[3];
     const uint32_t na = pk->hdr[4], nbq = pk->hdr[5], nh = pk->hdr[6], nl = pk->hdr[7];
@@ -272,34 +326,35 @@ extern "C" int bg_prove(bg_pk* pk, const uint32_t* witness, const uint32_t* rs, 
     // qap_h and then MSM H; MSMs A+L, B1 and B2 each run on their own stream once the
     // canonical scalars exist, so the latency-bound tails of one MSM overlap with the
     // accumulation of the others.
-    hipStream_t s0 = c.stream, s1 = c.aux[0], s2 = c.aux[1], s3 = c.aux[2];
-    hipEvent_t* ev = pk->ev;
-    if (!ev[0])
-      for (int i = 0; i < 16; i++) BG_HIP(hipEventCreate(&ev[i]));
+    hipStream_t s0 = sl.s[0], s1 = sl.s[1], s2 = sl.s[2], s3 = sl.s[3];
+    const int sched = prove_schedule();
+    if (sched == 5) s1 = s2 = s3 = s0;  // one stream: every kernel alone (profiling)
+    hipEvent_t* ev = sl.ev;
+    sl.ensure(nz, d);
     BG_HIP(hipEventRecord(ev[0], s0));
     // 1. witness upload (16-bit words are packed on the device)
-    uint32_t* z = pk->z.as<uint32_t>();
+    uint32_t* z = sl.z.as<uint32_t>();
     if (dev_witness) {
       BG_HIP(hipMemcpyAsync(z, witness, 32 * nz, hipMemcpyDeviceToDevice, s0));
     } else if (words16) {
-      BG_HIP(hipMemcpyAsync(pk->zraw.ptr, witness, 64 * nz, hipMemcpyHostToDevice, s0));
-      launch_pack16(pk->zraw.as<uint32_t>(), z, 8 * nz, s0);
+      BG_HIP(hipMemcpyAsync(sl.zraw.ptr, witness, 64 * nz, hipMemcpyHostToDevice, s0));
+      launch_pack16(sl.zraw.as<uint32_t>(), z, 8 * nz, s0);
     } else {
       BG_HIP(hipMemcpyAsync(z, witness, 32 * nz, hipMemcpyHostToDevice, s0));
     }
     BG_HIP(hipEventRecord(ev[1], s0));
     // 2. evaluation vectors and canonical scalars
-    uint32_t* zc = pk->zc.as<uint32_t>();
+    uint32_t* zc = sl.zc.as<uint32_t>();
     {
       ScopedKTime kt("evals", s0);
       k_from_mont<<<nb(nz, 256), 256, 0, s0>>>(z, zc, nz);
       BG_HIP(hipEventRecord(ev[2], s0));  // canonical scalars ready
       k_evals<<<nb(d, 256), 256, 0, s0>>>(pk->A.offs.as<uint32_t>(), pk->A.cols.as<uint32_t>(),
-                                          pk->A.coef.as<uint32_t>(), z, nc, ni, 1, d, pk->ea.as<uint32_t>());
+                                          pk->A.coef.as<uint32_t>(), z, nc, ni, 1, d, sl.ea.as<uint32_t>());
       k_evals<<<nb(d, 256), 256, 0, s0>>>(pk->B.offs.as<uint32_t>(), pk->B.cols.as<uint32_t>(),
-                                          pk->B.coef.as<uint32_t>(), z, nc, ni, 0, d, pk->eb.as<uint32_t>());
+                                          pk->B.coef.as<uint32_t>(), z, nc, ni, 0, d, sl.eb.as<uint32_t>());
       k_evals<<<nb(d, 256), 256, 0, s0>>>(pk->C.offs.as<uint32_t>(), pk->C.cols.as<uint32_t>(),
-                                          pk->C.coef.as<uint32_t>(), z, nc, ni, 0, d, pk->ec.as<uint32_t>());
+                                          pk->C.coef.as<uint32_t>(), z, nc, ni, 0, d, sl.ec.as<uint32_t>());
       BG_HIP(hipGetLastError());
     }
     BG_HIP(hipEventRecord(ev[3], s0));
@@ -311,26 +366,26 @@ extern "C" int bg_prove(bg_pk* pk, const uint32_t* witness, const uint32_t* rs, 
     //   2: as 0, but the G1 batch waits until B2's buckets are complete
     //   3: as 0, but the G1 batch waits for qap_h (default: the shortest at 2^18)
     //   4: qap_h first; then B2 on s3, the G1 batch A, B1, L on s1 and H on s0
-    const int sched = prove_schedule();
+    //   5: as 3 on a single stream (no overlap: per-kernel times for profiling)
     for (hipStream_t sx : {s1, s2, s3}) BG_HIP(hipStreamWaitEvent(sx, ev[2], 0));
     const uint32_t* g1p[4] = {pk->pa.as<uint32_t>(), pk->pb1.as<uint32_t>(), pk->pl.as<uint32_t>(),
                               pk->ph.as<uint32_t>()};
-    const uint32_t* g1s[4] = {zc, zc, zc + 8 * ni, pk->h.as<uint32_t>()};
+    const uint32_t* g1s[4] = {zc, zc, zc + 8 * ni, sl.h.as<uint32_t>()};
     const uint64_t g1n[4] = {na, nbq, nl, nh};
     auto run_qap = [&] {
       ScopedKTime kt("qap_h", s0);
-      qap_h_run(pk->ea.as<uint32_t>(), pk->eb.as<uint32_t>(), pk->ec.as<uint32_t>(), pk->h.as<uint32_t>(),
-                pk->tmp.as<uint32_t>(), (int)lg, true, s0);
+      qap_h_run(sl.ea.as<uint32_t>(), sl.eb.as<uint32_t>(), sl.ec.as<uint32_t>(), sl.h.as<uint32_t>(),
+                sl.tmp.as<uint32_t>(), (int)lg, true, s0);
       BG_HIP(hipEventRecord(ev[4], s0));
     };
     auto run_b2 = [&] {
       BG_HIP(hipEventRecord(ev[9], s3));
-      msm_enqueue(2, pk->pb2.as<uint32_t>(), zc, nbq, 0, s3, pk->work[2]);
+      msm_enqueue(2, pk->pb2.as<uint32_t>(), zc, nbq, 0, s3, sl.work[2]);
       BG_HIP(hipEventRecord(ev[10], s3));
     };
     auto run_g1_batch3 = [&] {
       BG_HIP(hipEventRecord(ev[5], s1));
-      msm_enqueue_batch(1, 3, g1p, g1s, g1n, 0, s1, pk->work[0]);
+      msm_enqueue_batch(1, 3, g1p, g1s, g1n, 0, s1, sl.work[0]);
       BG_HIP(hipEventRecord(ev[6], s1));
     };
     if (sched == 4) {
@@ -342,20 +397,20 @@ extern "C" int bg_prove(bg_pk* pk, const uint32_t* witness, const uint32_t* rs, 
     } else {
       run_b2();
       if (sched == 0 || sched == 2) {
-        if (sched == 2) BG_HIP(hipStreamWaitEvent(s1, pk->work[2].mid, 0));
+        if (sched == 2) BG_HIP(hipStreamWaitEvent(s1, sl.work[2].mid, 0));
         run_g1_batch3();
       }
       run_qap();
-      if (sched == 3) {
+      if (sched == 3 || sched == 5) {
         BG_HIP(hipStreamWaitEvent(s1, ev[4], 0));
         run_g1_batch3();
       }
     }
     if (sched != 1) {
-      msm_enqueue(1, pk->ph.as<uint32_t>(), pk->h.as<uint32_t>(), nh, 0, s0, pk->work[1]);
+      msm_enqueue(1, pk->ph.as<uint32_t>(), sl.h.as<uint32_t>(), nh, 0, s0, sl.work[1]);
     } else {
       BG_HIP(hipEventRecord(ev[5], s0));
-      msm_enqueue_batch(1, 4, g1p, g1s, g1n, 0, s0, pk->work[0]);
+      msm_enqueue_batch(1, 4, g1p, g1s, g1n, 0, s0, sl.work[0]);
       BG_HIP(hipEventRecord(ev[6], s0));
     }
     BG_HIP(hipEventRecord(ev[11], s0));
@@ -383,7 +438,7 @@ extern "C" int bg_prove(bg_pk* pk, const uint32_t* witness, const uint32_t* rs, 
     // 5. finish the MSMs and combine (groth16.bend:298-308); slots as prove.bend:64-116.
     //    Everything but the H term is combined while MSM H still runs on the device.
     uint32_t g1res[4 * 16], mB2[32], mHs[16];
-    msm_finish(pk->work[0], g1res);
+    msm_finish(sl.work[0], g1res);
     const uint32_t *mA = g1res, *mB1 = g1res + 16, *mL = g1res + 32;
     const uint32_t* mH = g1res + 48;
     bgh::P1 A = bgh::load_affine<bgh::Fq>(mA).add(r_d1).add(alpha);
@@ -391,11 +446,11 @@ extern "C" int bg_prove(bg_pk* pk, const uint32_t* witness, const uint32_t* rs, 
     bgh::P1 C0 = A.mul(sc).add(B1.mul(rc)).add(rs_d1.neg()).add(bgh::load_affine<bgh::Fq>(mL));
     uint32_t proof[64];
     bgh::store_affine(A, proof);
-    msm_finish(pk->work[2], mB2);
+    msm_finish(sl.work[2], mB2);
     bgh::P2 B2 = bgh::load_affine<bgh::Fq2>(mB2).add(s_d2).add(beta2);
     bgh::store_affine(B2, proof + 16);
     if (sched != 1) {
-      msm_finish(pk->work[1], mHs);
+      msm_finish(sl.work[1], mHs);
       mH = mHs;
     }
     bgh::P1 C = C0.add(bgh::load_affine<bgh::Fq>(mH));
@@ -421,8 +476,10 @@ extern "C" int bg_prove(bg_pk* pk, const uint32_t* witness, const uint32_t* rs, 
       }
       t_ms[8] = std::chrono::duration<double, std::milli>(t1 - t0).count();
     }
-  });
+  }
 }
+
+}  // namespace bg
 
 static void hex_be(const uint32_t* packed_mont, char* out) {
   // canonical big-endian "0x" + 64 hex digits (bend2/src/fq.bend:2482-2489)

@@ -1,0 +1,36 @@
+// Per-proof execution state and the prove driver shared by bg_prove and the proving
+// engine (bg_engine.hip).  A resident key (bg_pk) is read-only while proofs run, so any
+// number of slots may prove with one key at once.
+#pragma once
+#include "bg_runtime.hpp"
+#include "bg_msm.hpp"
+
+struct bg_pk;
+
+namespace bg {
+
+// One proof in flight: four streams (witness path + qap_h + H, G1 batch, spare, B2),
+// phase events and the device work arrays, grown to the largest key proved on it.
+struct ProveSlot {
+  hipStream_t s[4] = {};
+  hipEvent_t ev[16] = {};
+  DevArray z, zc, zraw, ea, eb, ec, h, tmp;
+  uint64_t cap_nz = 0, cap_d = 0;
+  MsmWork work[3];  // G1 batch (A, B1, L), H, B2
+  ProveSlot();
+  ~ProveSlot();
+  ProveSlot(const ProveSlot&) = delete;
+  ProveSlot& operator=(const ProveSlot&) = delete;
+  void ensure(uint64_t nz, uint64_t d);
+  void release_buffers();
+  uint64_t device_bytes() const;
+};
+
+// Device bytes one proof with this key needs in a fresh slot (work arrays + MSM work).
+uint64_t prove_bytes_estimate(const bg_pk* pk);
+
+// Runs one proof on slot sl (blocking until the proof is on the host).
+void prove_on(bg_pk* pk, ProveSlot& sl, const uint32_t* witness, const uint32_t* rs, uint32_t* proof_out, int flags,
+              double* t_ms);
+
+}  // namespace bg

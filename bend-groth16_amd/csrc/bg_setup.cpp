@@ -323,3 +323,92 @@ extern "C" int bg_setup_square_chain(const char* dir, int log2, uint64_t seed, i
     return BG_ERR_NOMEM;
   }
 }
+
+// ---------------------------------------------------------------- synthetic bases
+namespace {
+
+P1 g1_generator() { return P1::from_affine(Fq::one(), Fq::one() + Fq::one()); }
+
+P2 g2_generator() {
+  Fq2 x{Fq::to_mont(Fq::raw(0x46debd5cd992f6edULL, 0x674322d4f75edaddULL, 0x426a00665e5c4479ULL, 0x1800deef121f1e76ULL)),
+        Fq::to_mont(Fq::raw(0x97e485b7aef312c2ULL, 0xf1aa493335a9e712ULL, 0x7260bfb731fb5d25ULL, 0x198e9393920d483aULL))};
+  Fq2 y{Fq::to_mont(Fq::raw(0x4ce6cc0166fa7daaULL, 0xe3d1e7690c43d37bULL, 0x4aab71808dcb408fULL, 0x12c85ea5db8c6debULL)),
+        Fq::to_mont(Fq::raw(0x55acdadcd122975bULL, 0xbc4b313370b38ef3ULL, 0xec9e99ad690c3395ULL, 0x090689d0585ff075ULL))};
+  return P2::from_affine(x, y);
+}
+
+// affine Montgomery coordinates -> packed words (8 x u32 per Fq)
+void pack_fq(const Fq& x, uint32_t* o) { x.to_u32(o); }
+void pack_fq2(const Fq2& x, uint32_t* o) {
+  x.c0.to_u32(o);
+  x.c1.to_u32(o + 8);
+}
+
+int fixed_base_packed(int group, const std::vector<Fr>& ks, uint32_t* out, int threads) {
+  if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+  if (group == 1) {
+    std::vector<Fq> px, py;
+    fixed_base(g1_generator(), ks, px, py, threads);
+    for (size_t i = 0; i < ks.size(); i++) {
+      pack_fq(px[i], out + 16 * i);
+      pack_fq(py[i], out + 16 * i + 8);
+    }
+  } else {
+    std::vector<Fq2> px, py;
+    fixed_base(g2_generator(), ks, px, py, threads);
+    for (size_t i = 0; i < ks.size(); i++) {
+      pack_fq2(px[i], out + 32 * i);
+      pack_fq2(py[i], out + 32 * i + 16);
+    }
+  }
+  return BG_OK;
+}
+
+}  // namespace
+
+// k_i * G (G1 or G2 generator) for canonical packed scalars: the closed form of an MSM
+// over bases k_i G, and the generator of synthetic bases.
+extern "C" int bg_fixed_base_mul(int group, const uint32_t* scalars, uint64_t n, uint32_t* out_affine, int threads) {
+  try {
+    if ((group != 1 && group != 2) || (n && (!scalars || !out_affine))) {
+      bg::set_error("bg_fixed_base_mul: bad argument");
+      return BG_ERR_ARG;
+    }
+    std::vector<Fr> ks(n);
+    for (uint64_t i = 0; i < n; i++) {
+      const uint32_t* s = scalars + 8 * i;
+      ks[i] = Fr::to_mont(Fr::raw((uint64_t)s[0] | (uint64_t)s[1] << 32, (uint64_t)s[2] | (uint64_t)s[3] << 32,
+                                  (uint64_t)s[4] | (uint64_t)s[5] << 32, (uint64_t)s[6] | (uint64_t)s[7] << 32));
+    }
+    return fixed_base_packed(group, ks, out_affine, threads);
+  } catch (const std::bad_alloc&) {
+    bg::set_error("bg_fixed_base_mul: host allocation failed");
+    return BG_ERR_NOMEM;
+  }
+}
+
+// n bases k_i G with k_i uniform in [0, r) from the seeded generator; k_out (optional)
+// receives the k_i canonical packed.
+extern "C" int bg_gen_bases(int group, uint64_t seed, uint64_t n, uint32_t* out_affine, uint32_t* k_out, int threads) {
+  try {
+    if ((group != 1 && group != 2) || (n && !out_affine)) {
+      bg::set_error("bg_gen_bases: bad argument");
+      return BG_ERR_ARG;
+    }
+    Rng rng(seed);
+    std::vector<Fr> ks(n);
+    for (auto& k : ks) k = rng.fr();
+    if (k_out)
+      for (uint64_t i = 0; i < n; i++) {
+        Fr c = ks[i].from_mont();
+        for (int j = 0; j < 4; j++) {
+          k_out[8 * i + 2 * j] = (uint32_t)c.v[j];
+          k_out[8 * i + 2 * j + 1] = (uint32_t)(c.v[j] >> 32);
+        }
+      }
+    return fixed_base_packed(group, ks, out_affine, threads);
+  } catch (const std::bad_alloc&) {
+    bg::set_error("bg_gen_bases: host allocation failed");
+    return BG_ERR_NOMEM;
+  }
+}

@@ -109,7 +109,8 @@ int bg_pk_load(const char* dir, bg_pk** out);
 int bg_pk_free(bg_pk* pk);
 /* header: num_constraints, num_instance, num_witness, domain_log2, n_a, n_b, n_h, n_l */
 int bg_pk_header(const bg_pk* pk, uint32_t* header8);
-/* Device bytes held by a loaded key. */
+/* Device bytes held by a loaded key (the resident queries, matrices and NTT tables;
+ * per-proof work arrays live in the prover slots). */
 int bg_pk_device_bytes(const bg_pk* pk, uint64_t* bytes);
 
 /* One proof.  witness: num_instance + num_witness Fr (Montgomery, FORMAT.md witness.bin
@@ -123,6 +124,33 @@ int bg_pk_device_bytes(const bg_pk* pk, uint64_t* bytes);
 int bg_prove(bg_pk* pk, const uint32_t* witness, const uint32_t* rs, uint32_t* proof_out,
              int flags, double* t_ms);
 
+/* ------------------------------------------------------------------ proving engine
+ * Continuous batching of proofs on one device.  The reference proves one instance per
+ * process (bend2/prove.bend:129-162, run by bend2/scripts/prove.sh); the engine is the
+ * batched / served form of the same Pr.run: each request is proved exactly as bg_prove
+ * proves it alone, on one of `slots` worker slots (four streams each), so proofs of
+ * different keys and sizes overlap on the device.  Requests are taken in arrival order
+ * when their work arrays fit hbm_budget_bytes (0: 90 % of the device memory free at
+ * creation; resident keys are not counted).  Thread-safe. */
+typedef struct bg_engine bg_engine;
+int bg_engine_create(int slots, uint64_t hbm_budget_bytes, bg_engine** out);
+/* Queues one proof; witness / rs / flags as bg_prove (host inputs are copied; a device
+ * witness must stay valid until bg_engine_wait returns).  Fails with BG_ERR_NOMEM when
+ * one proof of this key exceeds the budget. */
+int bg_engine_submit(bg_engine* e, bg_pk* pk, const uint32_t* witness, const uint32_t* rs, int flags,
+                     uint64_t* ticket);
+/* Blocks until the proof is done; proof_out as bg_prove.  t_ms (optional, 3 doubles):
+ * time queued, time proving, submit-to-done latency. */
+int bg_engine_wait(bg_engine* e, uint64_t ticket, uint32_t* proof_out, double* t_ms);
+/* info: slots, device bytes reserved by the slots, budget, proofs completed. */
+int bg_engine_info(bg_engine* e, uint64_t* info4);
+/* Finishes every queued proof, then stops the workers. */
+int bg_engine_destroy(bg_engine* e);
+/* count proofs (key, witness, rs each) through a temporary engine with `slots` slots
+ * (<= 0: 4); proofs[i] as bg_prove's proof_out. */
+int bg_prove_batch(int count, bg_pk* const* pks, const uint32_t* const* witnesses, const uint32_t* const* rs,
+                   uint32_t* const* proofs, int flags, int slots);
+
 /* Proof as the reference prints it (bend2/FORMAT.md:33): "A x y\nB x0 x1 y0 y1\nC x y\n"
  * with canonical big-endian 0x hex; proof in BG_FMT_PACKED.  Returns bytes written. */
 int bg_proof_format(const uint32_t* proof_packed, char* buf, int cap);
@@ -133,6 +161,14 @@ int bg_proof_format(const uint32_t* proof_packed, char* buf, int cap);
  * FORMAT.md files under dir plus trapdoor.bin (tau, alpha, beta, gamma, delta as
  * Montgomery Fr) for closed-form checks.  threads <= 0 uses all host cores. */
 int bg_setup_square_chain(const char* dir, int log2, uint64_t seed, int threads);
+
+/* k_i * G for canonical packed scalars (G the G1 or G2 generator), affine packed out
+ * (host).  The closed form of an MSM over bases k_i G; no counterpart in the reference
+ * (its harness builds such bases with arkworks, ref/src/bench.rs). */
+int bg_fixed_base_mul(int group, const uint32_t* scalars, uint64_t n, uint32_t* out_affine, int threads);
+/* n synthetic bases k_i G with k_i uniform in [0, r) from the seeded generator (the
+ * primitive MSM inputs); k_out (optional) receives the k_i canonical packed. */
+int bg_gen_bases(int group, uint64_t seed, uint64_t n, uint32_t* out_affine, uint32_t* k_out, int threads);
 
 /* ------------------------------------------------------------------ diagnostics
  * Known-answer and element-wise kernels used by the parity tests (host buffers, packed). */

@@ -1,0 +1,111 @@
+"""Proving engine (continuous batching, bg_engine_* / bg_prove_batch): every proof made in a
+mix of concurrent requests of different sizes equals the proof of the same inputs made
+alone by bg_prove (itself pinned to the oracle prover and the trapdoor closed form in
+test_gpu_prove.py), bit for bit; a device-memory budget that admits one proof at a time
+changes nothing but the schedule."""
+import numpy as np
+import pytest
+
+import bendg
+from instance import closed_form, load, points_of
+
+pytestmark = pytest.mark.gpu
+
+SIZES = (4, 8, 10, 12)
+
+
+def _files(d):
+    z = np.fromfile(d / "witness.bin", dtype=np.uint32)
+    rs = np.fromfile(d / "rs.bin", dtype=np.uint32)
+    return z, rs
+
+
+@pytest.fixture(scope="module")
+def keys(tmp_path_factory):
+    out = {}
+    for lg in SIZES:
+        d = tmp_path_factory.mktemp(f"e{lg}")
+        bendg.setup_square_chain(d, lg, seed=900 + lg)
+        out[lg] = d
+    return out
+
+
+@pytest.fixture(scope="module")
+def loaded(bg, keys):
+    pks = {lg: bendg.ProvingKey(d) for lg, d in keys.items()}
+    solo = {}
+    for lg, pk in pks.items():
+        z, rs = _files(keys[lg])
+        solo[lg], _ = pk.prove(z, rs, fmt="words16")
+    yield pks, solo
+    for pk in pks.values():
+        pk.close()
+
+
+def test_solo_proofs_are_closed_form(loaded, keys):
+    _, solo = loaded
+    lg = 10
+    inst = load(keys[lg])
+    p = bendg.pack16(solo[lg].reshape(-1, 16)).reshape(-1).view(np.uint64)
+    eA, eB, eC = points_of(closed_form(inst))
+    assert np.array_equal(p[:8], eA) and np.array_equal(p[8:24], eB) and np.array_equal(p[24:32], eC)
+
+
+@pytest.mark.parametrize("slots", [1, 3, 6])
+def test_engine_mix_equals_solo(loaded, keys, slots):
+    pks, solo = loaded
+    rng = np.random.default_rng(slots)
+    order = [SIZES[i] for i in rng.integers(0, len(SIZES), size=24)]
+    eng = bendg.Engine(slots=slots)
+    files = {lg: _files(keys[lg]) for lg in SIZES}
+    print(f"engine slots={slots} order={order}", flush=True)
+    tickets = [(lg, eng.submit(pks[lg], *files[lg], fmt="words16")) for lg in order]
+    for lg, t in tickets:
+        proof, tm = eng.wait(t)
+        assert np.array_equal(proof, solo[lg]), f"K={lg} ticket {t}"
+        assert tm["latency"] >= tm["proving"] >= 0
+    info = eng.info()
+    assert info["completed"] == len(order) and info["slots"] == slots
+    eng.close()
+
+
+def test_engine_budget_admits_one_at_a_time(loaded, keys):
+    """A budget just above one K=12 proof's work arrays: requests wait for memory, idle
+    slots give their arrays back, and results are unchanged."""
+    pks, solo = loaded
+    eng0 = bendg.Engine(slots=1)
+    z, rs = _files(keys[12])
+    eng0.wait(eng0.submit(pks[12], z, rs))
+    need = eng0.info()["reserved_bytes"]
+    eng0.close()
+    eng = bendg.Engine(slots=4, hbm_budget=int(need * 1.3))
+    order = [12, 4, 12, 10, 12, 8, 4, 12]
+    tickets = [(lg, eng.submit(pks[lg], *_files(keys[lg]))) for lg in order]
+    for lg, t in tickets:
+        proof, _ = eng.wait(t)
+        assert np.array_equal(proof, solo[lg])
+    assert eng.info()["reserved_bytes"] <= int(need * 1.3) + need
+    eng.close()
+
+
+def test_engine_rejects_key_over_budget(loaded, keys):
+    pks, _ = loaded
+    eng = bendg.Engine(slots=2, hbm_budget=1 << 20)
+    z, rs = _files(keys[12])
+    with pytest.raises(bendg.BgError) as ei:
+        eng.submit(pks[12], z, rs)
+    assert ei.value.code == -3
+    eng.close()
+
+
+def test_prove_batch_packed(loaded, keys):
+    pks, solo = loaded
+    order = [10, 4, 12, 8, 10, 12]
+    ws, rr = [], []
+    for lg in order:
+        z, rs = _files(keys[lg])
+        ws.append(bendg.pack16(z.reshape(-1, 16)).reshape(-1))
+        rr.append(bendg.pack16(rs.reshape(-1, 16)).reshape(-1))
+    outs = bendg.prove_batch([pks[lg] for lg in order], ws, rr, fmt="packed", slots=3)
+    for lg, o in zip(order, outs):
+        assert np.array_equal(o, bendg.pack16(solo[lg].reshape(-1, 16)).reshape(-1))

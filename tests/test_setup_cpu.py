@@ -96,3 +96,25 @@ def test_setup_deterministic(tmp_path):
     c = tmp_path / "c"
     bendg.setup_square_chain(c, 3, seed=6)
     assert (a / "pk_a.bin").read_bytes() != (c / "pk_a.bin").read_bytes()
+
+
+@pytest.mark.parametrize("group", [1, 2])
+def test_fixed_base_and_synthetic_bases(group):
+    """bg_fixed_base_mul equals the oracle's k*G (affine group law in Python pins the
+    oracle); bg_gen_bases returns k_i G for the k_i it reports, the k_i below r."""
+    rng = np.random.default_rng(31 + group)
+    ks = O.rand_scalars(rng, 40)
+    ks[0] = 0
+    ks[1] = O.to_words(1)
+    ks[2] = O.to_words(R - 1)
+    got = bendg.fixed_base_mul(group, ks.view(np.uint32)).view(np.uint64)
+    gen = O.G1_GEN if group == 1 else O.G2_GEN
+    for i in (0, 1, 2, 7, 39):
+        k = O.from_words(ks[i])
+        want = O.g1_pack([O.g1_mul_aff(gen, k)])[0] if group == 1 else O.g2_pack([O.g2_mul_aff(gen, k)])[0]
+        assert np.array_equal(got[i], want), i
+    pts, k = bendg.gen_bases(group, 5, 64, with_k=True)
+    assert all(O.from_words(v) < R for v in k.view(np.uint64).reshape(-1, 4))
+    assert np.array_equal(pts, bendg.fixed_base_mul(group, k))
+    assert np.array_equal(pts, bendg.gen_bases(group, 5, 64))
+    assert not np.array_equal(pts, bendg.gen_bases(group, 6, 64))
