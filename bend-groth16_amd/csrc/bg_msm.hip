@@ -75,7 +75,9 @@ uint64_t msm_work_bytes(const MsmPlan& p, int group) {
                + (uint64_t)p.nb * xb             // buckets
                + 2ull * p.acc_threads * (8 + xb)  // partial slots
                + 4 * mcap * (8 + xb)             // merge levels
-               + 2 * ngroups * xb + 2 * nout * xb + (1u << 16);
+               + 2 * ngroups * xb + 2 * nout * xb + (1u << 16)
+               + 32 * p.n                         // reduced scalars
+               + 4ull * p.W * p.B * (32 * p.M + p.n / 16384 + 1);  // chunk histograms (bound)
   return b + b / 8;  // grow-only buffers keep 1/8 slack
 }
 
@@ -105,44 +107,11 @@ __device__ __forceinline__ uint32_t get_bits(const uint32_t* s, int lo, int c) {
   return v & ((1u << c) - 1u);
 }
 
-// Rank of this lane's entry inside bucket `key` (NOKEY: no entry), from the histogram
-// atomic.  Lanes of a wave that share the key of the first pending lane take a single
-// atomic between them (small or repeated scalars, or a short top window, send many
-// digits of a wave to one bucket); once the first pending key is unshared the rest take
-// one atomic each, so distinct keys cost one extra ballot.
-__device__ __forceinline__ uint32_t bucket_rank(uint32_t* __restrict__ counts, uint32_t key) {
-  const uint32_t lane = __lane_id();
-  const uint64_t below = (1ull << lane) - 1ull;
-  bool pending = key != NOKEY;
-  uint32_t rank = 0;
-  for (int round = 0; round < 8; round++) {
-    uint64_t pend = __ballot(pending);
-    if (!pend) break;
-    int leader = __ffsll((unsigned long long)pend) - 1;
-    uint32_t lkey = __shfl(key, leader);
-    bool mine = pending && key == lkey;
-    uint64_t same = __ballot(mine);
-    if (__popcll(same) == 1) break;
-    uint32_t base = 0;
-    if ((int)lane == leader) base = atomicAdd(&counts[lkey], (uint32_t)__popcll(same));
-    base = __shfl(base, leader);
-    if (mine) {
-      rank = base + (uint32_t)__popcll(same & below);
-      pending = false;
-    }
-  }
-  if (pending) rank = atomicAdd(&counts[key], 1u);
-  return rank;
-}
-
-// Signed c-bit digits of every scalar; for each nonzero digit the (bucket key, point
-// index | sign) pair and its rank inside the bucket (the value the histogram atomic
-// returns), so the scatter needs no atomics.
-// A base at infinity (all-zero affine encoding) contributes nothing and gets no entries,
-// so the accumulation never meets an infinite addend.
-__global__ void k_digits(const uint32_t* __restrict__ scalars, const uint32_t* __restrict__ points, int aff_words,
-                         uint32_t n, int c, int W, uint32_t key_base, uint32_t* __restrict__ keys,
-                         uint32_t* __restrict__ vals, uint32_t* __restrict__ ranks, uint32_t* __restrict__ counts) {
+// Reduced scalars: s mod r, or 0 when the base is the point at infinity (all-zero affine
+// encoding), so such a base gets no bucket entries and the accumulation never meets an
+// infinite addend.
+__global__ void k_prep_scalars(const uint32_t* __restrict__ scalars, const uint32_t* __restrict__ points,
+                               int aff_words, uint32_t n, uint32_t* __restrict__ out) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint32_t s[8];
@@ -151,39 +120,109 @@ __global__ void k_digits(const uint32_t* __restrict__ scalars, const uint32_t* _
   s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w;
   s[4] = b.x; s[5] = b.y; s[6] = b.z; s[7] = b.w;
   reduce_mod_r(s);
-  {
-    const uint4* pp = reinterpret_cast<const uint4*>(points + (uint64_t)aff_words * i);
-    uint32_t nz = 0;
-    for (int k = 0; k < aff_words / 4; k++) {
-      uint4 v = pp[k];
-      nz |= v.x | v.y | v.z | v.w;
-    }
-    if (!nz)
-      for (int k = 0; k < 8; k++) s[k] = 0;
+  const uint4* pp = reinterpret_cast<const uint4*>(points + (uint64_t)aff_words * i);
+  uint32_t nz = 0;
+  for (int k = 0; k < aff_words / 4; k++) {
+    uint4 v = pp[k];
+    nz |= v.x | v.y | v.z | v.w;
   }
+  uint4* o = reinterpret_cast<uint4*>(out + 8ull * i);
+  o[0] = nz ? make_uint4(s[0], s[1], s[2], s[3]) : make_uint4(0, 0, 0, 0);
+  o[1] = nz ? make_uint4(s[4], s[5], s[6], s[7]) : make_uint4(0, 0, 0, 0);
+}
+
+// Signed c-bit digits, window by window: digit_w = bits_w + carry_w, minus 2^c when it
+// exceeds 2^(c-1) (then carrying 1 into window w+1).  The carry into window w has a
+// closed form, so each window is computed on its own: carry_w = 1 exactly when the low
+// w*c bits of s exceed the constant with bit j*c + c-1 set for every j < w (the largest
+// value the digits below window w reach without a carry).
+__device__ __forceinline__ uint32_t window_carry(const uint32_t* s, const uint32_t* pat, int nbits) {
+  uint32_t gt = 0, decided = 0;
+#pragma unroll
+  for (int k = 7; k >= 0; k--) {
+    uint32_t m = nbits >= 32 * (k + 1) ? 0xFFFFFFFFu : (nbits <= 32 * k ? 0u : (1u << (nbits - 32 * k)) - 1u);
+    uint32_t x = s[k] & m, y = pat[k];
+    uint32_t diff = (x != y) & !decided;
+    gt |= diff & (x > y);
+    decided |= diff;
+  }
+  return gt;
+}
+
+// Counting sort of (bucket, point) entries without global atomics (radix-sort layout):
+// block (chunk, window, part) ranks its chunk's digits for one window in an LDS
+// histogram of up to HIST_MAX buckets, writes (key, index | sign, rank in block) for
+// each entry it owns and its histogram to hist[(window * nchunks + chunk) * B + bucket];
+// k_column_scan turns the histograms into per-chunk bucket bases and bucket totals.
+static constexpr uint32_t HIST_MAX = 16384;
+static constexpr int COUNT_T = 1024;
+
+__global__ __launch_bounds__(COUNT_T) void k_count(const uint32_t* __restrict__ sc, uint32_t n, int c,
+                                                   uint32_t chunk, uint32_t nchunks, uint32_t key_base,
+                                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                   uint32_t* __restrict__ ranks, uint32_t* __restrict__ hist_out) {
+  __shared__ uint32_t hist[HIST_MAX];
   const uint32_t B = 1u << (c - 1);
-  uint32_t carry = 0;
-  for (int w = 0; w < W; w++) {
-    uint32_t v = get_bits(s, w * c, c) + carry;
-    int32_t d;
-    if (v > B) {
-      d = (int32_t)v - (int32_t)(1u << c);
-      carry = 1;
-    } else {
-      d = (int32_t)v;
-      carry = 0;
-    }
-    uint32_t key = NOKEY, val = 0;
-    if (d != 0) {
-      uint32_t mag = d < 0 ? (uint32_t)(-d) : (uint32_t)d;
-      key = key_base + (uint32_t)w * B + mag - 1;
-      val = i | (d < 0 ? 0x80000000u : 0u);
-    }
-    uint32_t rank = bucket_rank(counts, key);
-    keys[(uint64_t)w * n + i] = key;
-    vals[(uint64_t)w * n + i] = val;
-    ranks[(uint64_t)w * n + i] = rank;
+  const uint32_t Bp = B < HIST_MAX ? B : HIST_MAX;
+  const uint32_t w = blockIdx.y;
+  const uint32_t lo_b = blockIdx.z * Bp;
+  const int nbits = (int)w * c;
+  uint32_t pat[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int j = 0; j < (int)w; j++) {
+    int pos = j * c + c - 1;
+    if (pos < 256) pat[pos >> 5] |= 1u << (pos & 31);
   }
+  for (uint32_t b = threadIdx.x; b < Bp; b += COUNT_T) hist[b] = 0;
+  __syncthreads();
+  const uint64_t i0 = (uint64_t)blockIdx.x * chunk;
+  const uint64_t i1 = min(i0 + chunk, (uint64_t)n);
+  const uint32_t full = 1u << c;
+  for (uint64_t i = i0 + threadIdx.x; i < i1; i += COUNT_T) {
+    uint32_t s[8];
+    const uint4* q = reinterpret_cast<const uint4*>(sc + 8 * i);
+    uint4 a = q[0], bq = q[1];
+    s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w;
+    s[4] = bq.x; s[5] = bq.y; s[6] = bq.z; s[7] = bq.w;
+    uint32_t v = (nbits < 256 ? get_bits(s, nbits, c) : 0u) + window_carry(s, pat, nbits);
+    bool owner = blockIdx.z == 0;
+    uint32_t key = NOKEY, val = 0, r = 0;
+    if (v != 0 && v != full) {
+      bool neg = v > B;
+      uint32_t bk = (neg ? full - v : v) - 1;
+      owner = bk - lo_b < Bp;
+      if (owner) {
+        key = key_base + w * B + bk;
+        val = (uint32_t)i | (neg ? 0x80000000u : 0u);
+        r = atomicAdd(&hist[bk - lo_b], 1u);
+      }
+    }
+    if (owner) {
+      uint64_t e = (uint64_t)w * n + i;
+      keys[e] = key;
+      vals[e] = val;
+      ranks[e] = r;
+    }
+  }
+  __syncthreads();
+  uint32_t* ho = hist_out + ((uint64_t)w * nchunks + blockIdx.x) * B + lo_b;
+  for (uint32_t b = threadIdx.x; b < Bp; b += COUNT_T) ho[b] = hist[b];
+}
+
+// For every (window, bucket) of one MSM: exclusive prefix of the chunk histograms in
+// place (the chunk's base inside the bucket) and the bucket total into counts.
+__global__ void k_column_scan(uint32_t* __restrict__ hist, uint32_t W, uint32_t B, uint32_t nchunks,
+                              uint32_t* __restrict__ counts) {
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (uint64_t)W * B) return;
+  uint32_t w = (uint32_t)(t / B), b = (uint32_t)(t % B);
+  uint32_t* col = hist + (uint64_t)w * nchunks * B + b;
+  uint32_t run = 0;
+  for (uint32_t k = 0; k < nchunks; k++) {
+    uint32_t x = col[(uint64_t)k * B];
+    col[(uint64_t)k * B] = run;
+    run += x;
+  }
+  counts[t] = run;
 }
 
 // exclusive scan, three kernels: 2048 elements per block
@@ -255,15 +294,21 @@ __global__ void k_zero_u32(uint32_t* __restrict__ out, uint64_t n) {
   if (i < n) out[i] = 0;
 }
 
-// sorted[offsets[key] + rank] = (key, val)
+// sorted[offsets[key] + chunk base + rank] = (key, val), one MSM's entries
 __global__ void k_scatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                          const uint32_t* __restrict__ ranks, uint64_t total, const uint32_t* __restrict__ offsets,
-                          uint2* __restrict__ sorted) {
+                          const uint32_t* __restrict__ ranks, uint32_t n, uint32_t W, uint32_t B, uint32_t chunk,
+                          uint32_t nchunks, uint32_t key_base, const uint32_t* __restrict__ hist,
+                          const uint32_t* __restrict__ offsets, uint2* __restrict__ sorted) {
   uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= total) return;
+  if (e >= (uint64_t)W * n) return;
   uint32_t key = keys[e];
   if (key == NOKEY) return;
-  sorted[offsets[key] + ranks[e]] = make_uint2(key, vals[e]);
+  uint32_t val = vals[e];
+  uint32_t kb = key - key_base;  // w * B + bucket
+  uint32_t w = kb / B, b = kb % B;
+  uint32_t ck = (val & 0x7FFFFFFFu) / chunk;
+  uint32_t base = hist[((uint64_t)w * nchunks + ck) * B + b];
+  sorted[offsets[key] + base + ranks[e]] = make_uint2(key, val);
 }
 
 __global__ void k_pack16(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint64_t nout) {
@@ -300,33 +345,68 @@ void launch_zero_u32(uint32_t* p, uint64_t n, hipStream_t s) {
   BG_HIP(hipGetLastError());
 }
 
+// chunk of scalars per counting block: at least 16384, at most 32 chunks per MSM
+static uint32_t count_chunk(uint64_t n) {
+  uint64_t c = (n + 31) / 32;
+  c = (c + 1023) / 1024 * 1024;
+  return (uint32_t)std::max<uint64_t>(16384, c);
+}
+
+uint64_t msm_hist_words(const MsmPlan& p, const uint64_t* ns) {
+  uint64_t t = 0;
+  for (uint32_t i = 0; i < p.M; i++)
+    if (ns[i]) t += (uint64_t)p.W * ((ns[i] + count_chunk(ns[i]) - 1) / count_chunk(ns[i])) * p.B;
+  return std::max<uint64_t>(t, 1);
+}
+
 // digits + counting sort of (bucket, point) pairs by bucket
 void msm_sort(const uint32_t* const* d_scalars, const uint32_t* const* d_points, const uint64_t* ns,
               const MsmPlan& p, hipStream_t s, int group, uint32_t* keys, uint32_t* vals, uint32_t* ranks,
-              uint32_t* counts, uint32_t* offsets, uint32_t* bsum, uint2* sorted, uint32_t* fix, uint64_t fix_words) {
+              uint32_t* counts, uint32_t* offsets, uint32_t* bsum, uint2* sorted, uint32_t* fix, uint64_t fix_words,
+              uint32_t* scal, uint32_t* hist) {
   uint32_t nblk = msm_scan_blocks(p.nb);
+  const uint32_t parts = p.B > HIST_MAX ? p.B / HIST_MAX : 1;
   {
-      ScopedKTime kt(group == 1 ? "msm_g1_digits" : "msm_g2_digits", s);
-      k_zero_u32<<<blocks_for(p.nb + 1, 256), 256, 0, s>>>(counts, (uint64_t)p.nb + 1);
-      k_zero_u32<<<blocks_for(fix_words, 256), 256, 0, s>>>(fix, fix_words);
-      uint64_t off = 0;
-      for (uint32_t i = 0; i < p.M; i++) {
-        if (ns[i])
-          k_digits<<<blocks_for(ns[i], 256), 256, 0, s>>>(d_scalars[i], d_points[i], group == 1 ? 16 : 32,
-                                                          (uint32_t)ns[i], p.c, p.W, i * p.kspan, keys + off,
-                                                          vals + off, ranks + off, counts);
-        off += (uint64_t)p.W * ns[i];
+    ScopedKTime kt(group == 1 ? "msm_g1_digits" : "msm_g2_digits", s);
+    k_zero_u32<<<blocks_for(p.nb + 1, 256), 256, 0, s>>>(counts, (uint64_t)p.nb + 1);
+    k_zero_u32<<<blocks_for(fix_words, 256), 256, 0, s>>>(fix, fix_words);
+    uint64_t off = 0, soff = 0, hoff = 0;
+    for (uint32_t i = 0; i < p.M; i++) {
+      const uint64_t n = ns[i];
+      if (n) {
+        const uint32_t ch = count_chunk(n), nch = (uint32_t)((n + ch - 1) / ch);
+        k_prep_scalars<<<blocks_for(n, 256), 256, 0, s>>>(d_scalars[i], d_points[i], group == 1 ? 16 : 32,
+                                                           (uint32_t)n, scal + soff);
+        k_count<<<dim3(nch, p.W, parts), COUNT_T, 0, s>>>(scal + soff, (uint32_t)n, p.c, ch, nch, i * p.kspan,
+                                                          keys + off, vals + off, ranks + off, hist + hoff);
+        k_column_scan<<<blocks_for((uint64_t)p.W * p.B, 256), 256, 0, s>>>(hist + hoff, p.W, p.B, nch,
+                                                                           counts + (uint64_t)i * p.kspan);
+        hoff += (uint64_t)p.W * nch * p.B;
       }
-      BG_HIP(hipGetLastError());
+      off += (uint64_t)p.W * n;
+      soff += 8 * n;
     }
-    {
-      ScopedKTime kt(group == 1 ? "msm_g1_sort" : "msm_g2_sort", s);
-      k_scan_local<<<nblk, SCAN_T, 0, s>>>(counts, offsets, bsum, p.nb);
-      k_scan_top<<<1, SCAN_T, 0, s>>>(bsum, nblk, offsets + p.nb);
-      k_scan_add<<<blocks_for(p.nb, 256), 256, 0, s>>>(offsets, bsum, p.nb);
-      k_scatter<<<blocks_for(p.entries, 256), 256, 0, s>>>(keys, vals, ranks, p.entries, offsets, sorted);
-      BG_HIP(hipGetLastError());
+    BG_HIP(hipGetLastError());
+  }
+  {
+    ScopedKTime kt(group == 1 ? "msm_g1_sort" : "msm_g2_sort", s);
+    k_scan_local<<<nblk, SCAN_T, 0, s>>>(counts, offsets, bsum, p.nb);
+    k_scan_top<<<1, SCAN_T, 0, s>>>(bsum, nblk, offsets + p.nb);
+    k_scan_add<<<blocks_for(p.nb, 256), 256, 0, s>>>(offsets, bsum, p.nb);
+    uint64_t off = 0, hoff = 0;
+    for (uint32_t i = 0; i < p.M; i++) {
+      const uint64_t n = ns[i];
+      if (n) {
+        const uint32_t ch = count_chunk(n), nch = (uint32_t)((n + ch - 1) / ch);
+        k_scatter<<<blocks_for((uint64_t)p.W * n, 256), 256, 0, s>>>(keys + off, vals + off, ranks + off,
+                                                                     (uint32_t)n, p.W, p.B, ch, nch, i * p.kspan,
+                                                                     hist + hoff, offsets, sorted);
+        hoff += (uint64_t)p.W * nch * p.B;
+      }
+      off += (uint64_t)p.W * n;
     }
+    BG_HIP(hipGetLastError());
+  }
 }
 
 void msm_enqueue_batch(int group, uint32_t M, const uint32_t* const* d_points, const uint32_t* const* d_scalars,

@@ -40,7 +40,7 @@ MsmPlan msm_plan_batch(const uint64_t* ns, uint32_t M, int group, int window_bit
 // Work buffers of one in-flight MSM (reused across calls).
 struct MsmWork {
   DevBuf keys, vals, ranks, counts, sorted, buckets, pkeys, pflags, ppts, mkeys, mflags, mpts, lvlS, lvlT, outPQ,
-      scan_tmp, fix, rfix, ptrs;
+      scan_tmp, fix, rfix, ptrs, scal, hist;
   HostBuf host_out, host_ptrs;
   MsmPlan plan;
   int group = 1;
@@ -49,13 +49,13 @@ struct MsmWork {
   uint64_t bytes() const {
     uint64_t t = 0;
     for (const DevBuf* b : {&keys, &vals, &ranks, &counts, &sorted, &buckets, &pkeys, &pflags, &ppts, &mkeys, &mflags,
-                            &mpts, &lvlS, &lvlT, &outPQ, &scan_tmp, &fix, &rfix, &ptrs})
+                            &mpts, &lvlS, &lvlT, &outPQ, &scan_tmp, &fix, &rfix, &ptrs, &scal, &hist})
       t += b->cap;
     return t;
   }
   void release() {
     for (DevBuf* b : {&keys, &vals, &ranks, &counts, &sorted, &buckets, &pkeys, &pflags, &ppts, &mkeys, &mflags,
-                      &mpts, &lvlS, &lvlT, &outPQ, &scan_tmp, &fix, &rfix, &ptrs})
+                      &mpts, &lvlS, &lvlT, &outPQ, &scan_tmp, &fix, &rfix, &ptrs, &scal, &hist})
       b->release();
     host_out.release();
     host_ptrs.release();
@@ -64,6 +64,9 @@ struct MsmWork {
     done = mid = nullptr;
   }
 };
+
+// Words of the per-chunk bucket histograms of the counting sort for this batch.
+uint64_t msm_hist_words(const MsmPlan& p, const uint64_t* ns);
 
 // Device bytes the work buffers of a batch with this plan occupy (upper estimate).
 uint64_t msm_work_bytes(const MsmPlan& p, int group);
@@ -93,7 +96,8 @@ uint32_t msm_scan_blocks(uint32_t n);
 void launch_zero_u32(uint32_t* p, uint64_t n, hipStream_t s);
 void msm_sort(const uint32_t* const* d_scalars, const uint32_t* const* d_points, const uint64_t* ns,
               const MsmPlan& p, hipStream_t s, int group, uint32_t* keys, uint32_t* vals, uint32_t* ranks,
-              uint32_t* counts, uint32_t* offsets, uint32_t* bsum, uint2* sorted, uint32_t* fix, uint64_t fix_words);
+              uint32_t* counts, uint32_t* offsets, uint32_t* bsum, uint2* sorted, uint32_t* fix, uint64_t fix_words,
+              uint32_t* scal, uint32_t* hist);
 void enqueue_g1(uint32_t M, const uint32_t* const* d_points, const uint32_t* const* d_scalars, const uint64_t* ns,
                 int window_bits, hipStream_t s, MsmWork& w);
 void enqueue_g2(uint32_t M, const uint32_t* const* d_points, const uint32_t* const* d_scalars, const uint64_t* ns,

@@ -368,7 +368,8 @@ void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* 
   } else {
     BG_HIP(hipMemcpyAsync(dptrs, hptrs, 8ull * M, hipMemcpyHostToDevice, s));
     msm_sort(d_scalars, d_points, ns, p, s, group, keys, vals, ranks, counts, offsets, bsum, sorted, fix,
-             1 + (uint64_t)p.nb);
+             1 + (uint64_t)p.nb, w.scal.as<uint32_t>(std::max<uint64_t>(8 * n, 8)),
+             w.hist.as<uint32_t>(msm_hist_words(p, ns)));
     {
       ScopedKTime kt(group == 1 ? "msm_g1_accumulate" : "msm_g2_accumulate", s);
       k_accumulate<F><<<blocks_of(p.acc_threads, 256), 256, 0, s>>>(dptrs, p.kspan, sorted, offsets + p.nb, p.L,
