@@ -46,8 +46,10 @@ MsmPlan msm_plan_batch(const uint64_t* ns, uint32_t M, int group, int window_bit
   p.B = 1u << (c - 1);
   p.kspan = (uint32_t)p.W * p.B;
   p.nb = M * p.kspan;
+  // buckets per level-1 group: 2, 4 or 8 (the four-lane level-1 sums need m <= 8)
   uint32_t mm = (uint32_t)tune_env("BG16_MSM_M");
-  p.m = std::min<uint32_t>(mm ? mm : 8, p.B);
+  mm = mm >= 8 || mm == 0 ? 8 : (mm >= 4 ? 4 : 2);
+  p.m = std::min<uint32_t>(mm, p.B);
   p.G = p.B / p.m;
   uint32_t kmax = group == 1 ? 256 : 128;
   uint32_t kk = (uint32_t)tune_env("BG16_MSM_K");
@@ -77,6 +79,7 @@ uint64_t msm_work_bytes(const MsmPlan& p, int group) {
                + 4 * mcap * (8 + xb)             // merge levels
                + 2 * ngroups * xb + 2 * nout * xb + (1u << 16)
                + 32 * p.n                         // reduced scalars
+               + 4 * (ngroups + 256) * xb          // level-1 role sums
                + 4ull * p.W * p.B * (32 * p.M + p.n / 16384 + 1);  // chunk histograms (bound)
   return b + b / 8;  // grow-only buffers keep 1/8 slack
 }

@@ -40,7 +40,7 @@ MsmPlan msm_plan_batch(const uint64_t* ns, uint32_t M, int group, int window_bit
 // Work buffers of one in-flight MSM (reused across calls).
 struct MsmWork {
   DevBuf keys, vals, ranks, counts, sorted, buckets, pkeys, pflags, ppts, mkeys, mflags, mpts, lvlS, lvlT, outPQ,
-      scan_tmp, fix, rfix, ptrs, scal, hist;
+      scan_tmp, fix, rfix, ptrs, scal, hist, mq;
   HostBuf host_out, host_ptrs;
   MsmPlan plan;
   int group = 1;
@@ -49,13 +49,13 @@ struct MsmWork {
   uint64_t bytes() const {
     uint64_t t = 0;
     for (const DevBuf* b : {&keys, &vals, &ranks, &counts, &sorted, &buckets, &pkeys, &pflags, &ppts, &mkeys, &mflags,
-                            &mpts, &lvlS, &lvlT, &outPQ, &scan_tmp, &fix, &rfix, &ptrs, &scal, &hist})
+                            &mpts, &lvlS, &lvlT, &outPQ, &scan_tmp, &fix, &rfix, &ptrs, &scal, &hist, &mq})
       t += b->cap;
     return t;
   }
   void release() {
     for (DevBuf* b : {&keys, &vals, &ranks, &counts, &sorted, &buckets, &pkeys, &pflags, &ppts, &mkeys, &mflags,
-                      &mpts, &lvlS, &lvlT, &outPQ, &scan_tmp, &fix, &rfix, &ptrs, &scal, &hist})
+                      &mpts, &lvlS, &lvlT, &outPQ, &scan_tmp, &fix, &rfix, &ptrs, &scal, &hist, &mq})
       b->release();
     host_out.release();
     host_ptrs.release();

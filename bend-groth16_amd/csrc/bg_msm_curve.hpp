@@ -222,16 +222,81 @@ __global__ __launch_bounds__(MERGE_T) void k_merge_level(const uint32_t* __restr
   }
 }
 
-// Level 1: thread g sums buckets i = j*m + l (l < m) of window w:
-//   S = sum_l b_l,  T = sum_l l * b_l   (running sums from the top)
+// Level 1: group g sums the m = 2^logm buckets i = j*m + l (l < m) of window w:
+//   S = sum_l b_l,  T = sum_l l * b_l.
+// A block of four waves takes 64 groups, one role per wave (roles are wave-uniform, so
+// no wave runs two of them): wave q < logm sums M_q = the b_l with bit q of l set, wave
+// logm sums N = the b_l with l < m/2 (partial sums through the scratch mq); then wave 0
+// forms T = sum_q 2^q M_q (Horner) and wave 1 forms S = M_{logm-1} + N.  Depth m/2 +
+// 2(logm - 1) additions instead of a 2m-long running sum.
 // CAREFUL = false uses the generic add with infinity selected around it and lists the
 // group in rl[1 ..] (count rl[0]) when two partial sums share an x coordinate;
 // CAREFUL = true recomputes only the listed groups with the exact add.
 template <class F, bool CAREFUL>
 __global__ __launch_bounds__(256) void k_reduce1(const uint32_t* __restrict__ buckets,
-                                                 const uint32_t* __restrict__ offsets, uint32_t B, uint32_t m,
+                                                 const uint32_t* __restrict__ offsets, uint32_t B, uint32_t logm,
                                                  uint32_t ngroups, uint32_t* __restrict__ lvlS,
-                                                 uint32_t* __restrict__ lvlT, uint32_t* __restrict__ rl) {
+                                                 uint32_t* __restrict__ lvlT, uint32_t* mq,
+                                                 uint32_t* __restrict__ rl) {
+  using C = Curve<F>;
+  using P = typename C::P;
+  __shared__ uint32_t shbad[64];
+  const uint64_t xw = C::xyzz_words;
+  const uint32_t m = 1u << logm, G = B >> logm;
+  const uint32_t q = threadIdx.x >> 6, gl = threadIdx.x & 63;  // role = wave
+  const uint32_t count = CAREFUL ? rl[0] : ngroups;
+  const uint64_t mstride = (uint64_t)gridDim.x * 64;  // scratch slots per role
+  for (uint32_t g0 = blockIdx.x * 64; g0 < count; g0 += gridDim.x * 64) {
+    const uint32_t idx = g0 + gl;
+    const bool valid = idx < count;
+    const uint32_t g = valid ? (CAREFUL ? rl[1 + idx] : idx) : 0;
+    const uint32_t w = g / G, j = g % G;
+    const uint64_t base = (uint64_t)w * B + (uint64_t)j * m;
+    const uint64_t slot = (uint64_t)blockIdx.x * 64 + gl;
+    uint32_t bad = 0;
+    if (q == 0) shbad[gl] = 0;
+    if (valid && q <= logm) {
+      P acc = C::infinity();
+      for (uint32_t l = 0; l < m; l++) {
+        const bool take = q < logm ? ((l >> q) & 1u) != 0 : l < m / 2;
+        if (!take) continue;
+        const uint64_t key = base + l;
+        if (offsets[key + 1] > offsets[key]) {
+          P bk = C::load(buckets + xw * key);
+          acc = CAREFUL ? C::add(acc, bk) : C::add_lite(acc, bk, bad);
+        }
+      }
+      C::store(mq + xw * (q * mstride + slot), acc);
+    }
+    __syncthreads();
+    if (valid && q == 0) {
+      P T = C::load(mq + xw * ((logm - 1) * mstride + slot));
+      for (int qq = (int)logm - 2; qq >= 0; qq--) {
+        T = C::dbl(T);
+        P mqq = C::load(mq + xw * (qq * mstride + slot));
+        T = CAREFUL ? C::add(T, mqq) : C::add_lite(T, mqq, bad);
+      }
+      C::store(lvlT + xw * g, T);
+    } else if (valid && q == 1) {
+      P hi = C::load(mq + xw * ((logm - 1) * mstride + slot));
+      P lo = C::load(mq + xw * (logm * mstride + slot));
+      C::store(lvlS + xw * g, CAREFUL ? C::add(hi, lo) : C::add_lite(hi, lo, bad));
+    }
+    if (bad) shbad[gl] = 1;
+    __syncthreads();
+    if (!CAREFUL && valid && q == 0 && shbad[gl]) rl[1 + atomicAdd(&rl[0], 1u)] = g;
+    __syncthreads();
+  }
+}
+
+// Level 1 with one lane per group (running sums from the top bucket, depth 2m): used
+// for G2, whose additions hold a whole register file per lane, so four role waves per
+// group would not fit the device in one round.
+template <class F, bool CAREFUL>
+__global__ __launch_bounds__(256) void k_reduce1_serial(const uint32_t* __restrict__ buckets,
+                                                        const uint32_t* __restrict__ offsets, uint32_t B,
+                                                        uint32_t m, uint32_t ngroups, uint32_t* __restrict__ lvlS,
+                                                        uint32_t* __restrict__ lvlT, uint32_t* __restrict__ rl) {
   using C = Curve<F>;
   uint32_t count = CAREFUL ? rl[0] : ngroups;
   for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < count; idx += gridDim.x * blockDim.x) {
@@ -294,12 +359,15 @@ __global__ __launch_bounds__(256) void k_reduce2(const uint32_t* __restrict__ lv
     __syncthreads();
     if (k == 0) shA[0] = C::infinity();
     __syncthreads();
-    for (uint32_t s = K / 2; s > 0; s >>= 1) {
-      if (k < s) {
-        shA[k] = CAREFUL ? C::add(shA[k], shA[k + s]) : C::add_lite(shA[k], shA[k + s], bad);
-        shU[k] = CAREFUL ? C::add(shU[k], shU[k + s]) : C::add_lite(shU[k], shU[k + s], bad);
+    // the two tree sums side by side: the first half of the block reduces shA, the
+    // second half shU (wave-uniform roles once K/2 >= 64)
+    if (K >= 2) {
+      const uint32_t half = K / 2, t = k % half;
+      P* arr = k < half ? shA : shU;
+      for (uint32_t s = half; s > 0; s >>= 1) {
+        if (t < s) arr[t] = CAREFUL ? C::add(arr[t], arr[t + s]) : C::add_lite(arr[t], arr[t + s], bad);
+        __syncthreads();
       }
-      __syncthreads();
     }
     if (k == 0) {
       P V = shA[0];
@@ -355,6 +423,7 @@ void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* 
   uint64_t ngroups = (uint64_t)p.M * p.W * p.G;
   uint32_t* lvlS = w.lvlS.as<uint32_t>(ngroups * xw);
   uint32_t* lvlT = w.lvlT.as<uint32_t>(ngroups * xw);
+  uint32_t* mq = w.mq.as<uint32_t>(4 * std::max<uint64_t>((ngroups + 63) / 64 * 64, 256) * xw);
   uint64_t nout = (uint64_t)p.M * p.W * p.H;
   uint64_t* hptrs = static_cast<uint64_t*>(w.host_ptrs.get(8ull * M));
   for (uint32_t i = 0; i < M; i++) hptrs[i] = reinterpret_cast<uint64_t>(d_points[i]);
@@ -411,9 +480,19 @@ void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* 
       launch_zero_u32(rl2, 1, s);
       {
         ScopedKTime kt(group == 1 ? "msm_g1_reduce1" : "msm_g2_reduce1", s);
-        k_reduce1<F, false><<<blocks_of(ngroups, 256), 256, 0, s>>>(buckets, offsets, p.B, p.m, (uint32_t)ngroups,
-                                                                     lvlS, lvlT, rl1);
-        k_reduce1<F, true><<<16, 64, 0, s>>>(buckets, offsets, p.B, p.m, (uint32_t)ngroups, lvlS, lvlT, rl1);
+        if (sizeof(typename C::P) > 128) {
+          k_reduce1_serial<F, false><<<blocks_of(ngroups, 256), 256, 0, s>>>(buckets, offsets, p.B, p.m,
+                                                                              (uint32_t)ngroups, lvlS, lvlT, rl1);
+          k_reduce1_serial<F, true><<<16, 64, 0, s>>>(buckets, offsets, p.B, p.m, (uint32_t)ngroups, lvlS, lvlT,
+                                                      rl1);
+        } else {
+          // one wave per role, 64 groups per block
+          const unsigned r1b = (unsigned)((ngroups + 63) / 64);
+          k_reduce1<F, false><<<r1b, 256, 0, s>>>(buckets, offsets, p.B, (uint32_t)log2m, (uint32_t)ngroups, lvlS,
+                                                  lvlT, mq, rl1);
+          k_reduce1<F, true><<<4, 256, 0, s>>>(buckets, offsets, p.B, (uint32_t)log2m, (uint32_t)ngroups, lvlS,
+                                               lvlT, mq, rl1);
+        }
       }
       ScopedKTime kt(group == 1 ? "msm_g1_reduce2" : "msm_g2_reduce2", s);
       k_reduce2<F, false><<<nblocks, p.K, smem, s>>>(lvlS, lvlT, log2m, outPQ, rl2);
