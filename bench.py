@@ -123,17 +123,39 @@ def standalone_msm_pass(key: Path, hdr, reps: int) -> dict:
     return out
 
 
+def probe_peak() -> float:
+    """Measured issue rate of v_mad_u64_u32 (16 independent chains per lane, four waves
+    per SIMD), best of two ~5 ms launches: the binding resource of the MSM kernels."""
+    return max(bendg.probe_rate(0, 65536)[0] for _ in range(2))
+
+
+def traffic_from_profile() -> dict | None:
+    """HBM bytes per launch of the G1 accumulation kernel, from the committed rocprofv3
+    --pmc summary (FETCH_SIZE + WRITE_SIZE scaled by a device copy of known size)."""
+    f = ROOT / "profiles" / "msm_g1_accumulate_2^19_traffic.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return {"bytes": float(line.split(":")[1]), "source": f"profiles/{f.name}"}
+    return None
+
+
 def roofline(standalone: dict, peak_mul: float) -> dict:
-    """Roofline of the G1 accumulation kernel (the MSMs' dominant kernel) from its
-    standalone average launch duration: one mixed add (10 Fq products x 136 multiplies)
-    per sorted (bucket, point) entry."""
+    """Roofline of the G1 accumulation kernel (the proof's dominant kernel) from its
+    standalone average launch duration (HIP events on the library stream): one mixed add
+    (10 Fq products x 136 multiplies) per sorted (bucket, point) entry."""
     k = standalone["msm_g1_accumulate"]
     entries = bendg.msm_plan_info(k["n"], 1)["entries"]
     work = entries * FQ_PER_MADD[1] * MULS_PER_FQ
     achieved = work / (k["avg_ms"] * 1e-3) / 1e12
-    return {"bound": "valu", "kernel": "msm_g1_accumulate", "achieved": round(achieved, 3),
+    tr = traffic_from_profile()
+    return {"bound": "valu", "kernel": "k_accumulate<G1> (msm_g1_accumulate)", "achieved": round(achieved, 3),
             "peak": round(peak_mul / 1e12, 3), "unit": "T int32-mul/s", "frac": round(achieved / (peak_mul / 1e12), 4),
-            "traffic": None, "work_per_launch": work, "points_per_launch": k["n"], "avg_launch_ms": k["avg_ms"]}
+            "traffic": tr["bytes"] if tr else None, "traffic_source": tr["source"] if tr else None,
+            "algorithmic_bytes": entries * (8 + 64), "work_per_launch": work, "points_per_launch": k["n"],
+            "entries_per_launch": entries, "avg_launch_ms": k["avg_ms"],
+            "peak_note": "measured v_mad_u64_u32 issue rate (spec VALU rate 256 CU x 64 lanes x 2.4 GHz = 39.3 T/s)"}
 
 
 def cpu_baseline(key: Path) -> dict:
@@ -451,6 +473,30 @@ def primitives_mode(a) -> dict:
             "config": {"workload": "G1 MSM 2^20/2^22, G2 MSM 2^20, Fr NTT 2^20/2^24"}}
 
 
+def roofline_mode(a) -> dict:
+    """The default line's standalone pass alone (profiled by rocprofv3 for the committed
+    roofline summary), plus a 256 MiB device copy whose counters calibrate FETCH_SIZE /
+    WRITE_SIZE."""
+    key = ensure_key(a.k, a.seed, Path(a.keys))
+    bendg.init(0)
+    pk = bendg.ProvingKey(key)
+    hdr = pk.header
+    pk.close()
+    hip = Hip()
+    nb = 256 << 20
+    src, dst = hip.alloc(nb), hip.alloc(nb)
+    hip.L.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    assert hip.L.hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), nb, 3) == 0
+    hip.sync()
+    hip.free(src)
+    hip.free(dst)
+    peak = probe_peak()
+    alone = standalone_msm_pass(key, hdr, 5)
+    return {"metric": "roofline of the G1 accumulation kernel", "value": roofline(alone, peak)["frac"],
+            "unit": "fraction of measured peak", "roofline": roofline(alone, peak),
+            "kernel_ms_standalone": {k: v["avg_ms"] for k, v in alone.items()}, "copy_bytes": nb}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -460,7 +506,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--keys", default=os.environ.get("BG16_KEYS", "/tmp/bg16_keys"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve", "primitives"])
+    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve", "primitives", "roofline"])
     ap.add_argument("--slots", type=lambda v: [int(x) for x in v.split(",")], default=[4, 8],
                     help="engine slots (batch: comma list to sweep)")
     ap.add_argument("--rates", type=lambda v: [float(x) for x in v.split(",")], default=[100.0, 200.0, 300.0],
@@ -479,7 +525,8 @@ def main() -> int:
     if hwq:
         os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, max(1, hwq)))
     if a.mode != "single":
-        line = {"batch": batch_mode, "serve": serve_mode, "primitives": primitives_mode}[a.mode](a)
+        line = {"batch": batch_mode, "serve": serve_mode, "primitives": primitives_mode,
+                "roofline": roofline_mode}[a.mode](a)
         print(json.dumps(line), flush=True)
         return 0
 
@@ -528,7 +575,7 @@ def main() -> int:
         return 1
     ms = elapsed * 1e3 / a.steps
     per_proof = {k: round(v[0] * v[1] / a.steps, 4) for k, v in sorted(timing.items(), key=lambda kv: -kv[1][0] * kv[1][1])}
-    peak, _ = bendg.probe_rate(0, 4096)
+    peak = probe_peak()
     log("standalone MSM pass")
     alone = standalone_msm_pass(key, hdr, 5)
     for k, v in sorted(alone.items()):

@@ -440,10 +440,13 @@ void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* 
              1 + (uint64_t)p.nb, w.scal.as<uint32_t>(std::max<uint64_t>(8 * n, 8)),
              w.hist.as<uint32_t>(msm_hist_words(p, ns)));
     {
-      ScopedKTime kt(group == 1 ? "msm_g1_accumulate" : "msm_g2_accumulate", s);
-      k_accumulate<F><<<blocks_of(p.acc_threads, 256), 256, 0, s>>>(dptrs, p.kspan, sorted, offsets + p.nb, p.L,
-                                                                     p.acc_threads, buckets, pkeys, pflags, ppts,
-                                                                     p.nb, fix);
+      {
+        ScopedKTime kt(group == 1 ? "msm_g1_accumulate" : "msm_g2_accumulate", s);
+        k_accumulate<F><<<blocks_of(p.acc_threads, 256), 256, 0, s>>>(dptrs, p.kspan, sorted, offsets + p.nb, p.L,
+                                                                       p.acc_threads, buckets, pkeys, pflags, ppts,
+                                                                       p.nb, fix);
+      }
+      ScopedKTime kt(group == 1 ? "msm_g1_merge" : "msm_g2_merge", s);
       {
         const uint32_t* ik = pkeys;
         const uint32_t* ifl = pflags;
