@@ -21,6 +21,12 @@ static int tune_env(const char* name) {
   return e ? atoi(e) : 0;
 }
 
+uint64_t accumulate_threads_g1();
+uint64_t accumulate_threads_g2();
+uint64_t accumulate_threads_per_round(int group) {
+  return group == 1 ? accumulate_threads_g1() : accumulate_threads_g2();
+}
+
 MsmPlan msm_plan_batch(const uint64_t* ns, uint32_t M, int group, int window_bits) {
   MsmPlan p;
   if (window_bits <= 0) window_bits = tune_env(group == 1 ? "BG16_MSM_C_G1" : "BG16_MSM_C_G2");
@@ -57,9 +63,19 @@ MsmPlan msm_plan_batch(const uint64_t* ns, uint32_t M, int group, int window_bit
   p.K = std::min<uint32_t>(kmax, p.G);
   p.H = p.G / p.K;
   p.entries = (uint64_t)p.W * n;
-  uint64_t L = p.entries / 16384;
-  uint32_t lmax = (uint32_t)tune_env("BG16_MSM_L");
-  p.L = (uint32_t)std::max<uint64_t>(2, std::min<uint64_t>(lmax ? lmax : 16, L));
+  // entries per accumulation thread: about 20, rounded so that the launch fills a whole
+  // number of device rounds (threads resident at once), which leaves no tail of a few
+  // waves; longer slices also leave fewer partial runs to merge
+  uint32_t lenv = (uint32_t)tune_env("BG16_MSM_L");
+  uint64_t L;
+  if (lenv) {
+    L = lenv;
+  } else {
+    const uint64_t cap = accumulate_threads_per_round(group);
+    const uint64_t k = std::max<uint64_t>(1, p.entries / (20 * cap));
+    L = (p.entries + k * cap - 1) / (k * cap);
+  }
+  p.L = (uint32_t)std::max<uint64_t>(2, std::min<uint64_t>(64, L));
   p.acc_threads = (uint32_t)((p.entries + p.L - 1) / p.L);
   return p;
 }

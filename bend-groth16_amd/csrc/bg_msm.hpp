@@ -65,6 +65,10 @@ struct MsmWork {
   }
 };
 
+// Accumulation threads of one group resident on the device at once (CUs x SIMDs x the
+// waves per SIMD that the accumulation kernel's registers allow).
+uint64_t accumulate_threads_per_round(int group);
+
 // Words of the per-chunk bucket histograms of the counting sort for this batch.
 uint64_t msm_hist_words(const MsmPlan& p, const uint64_t* ns);
 

@@ -384,6 +384,26 @@ __global__ __launch_bounds__(256) void k_reduce2(const uint32_t* __restrict__ lv
 }
 
 template <class F>
+uint64_t accumulate_round_threads() {
+  static uint64_t cached = [] {
+    int cus = 256;
+    int wps = 2;
+    hipFuncAttributes attr;
+    try {
+      cus = ctx().cu_count;
+      if (hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(&k_accumulate<F>)) == hipSuccess &&
+          attr.numRegs > 0) {
+        int regs = (attr.numRegs + 7) / 8 * 8;
+        wps = std::max(1, std::min(8, 512 / regs));
+      }
+    } catch (const Error&) {
+    }
+    return (uint64_t)cus * 4 * wps * 64;
+  }();
+  return cached;
+}
+
+template <class F>
 void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* const* d_scalars, const uint64_t* ns,
                    int window_bits, hipStream_t s, MsmWork& w, int group) {
   using C = Curve<F>;

@@ -6,4 +6,5 @@ void enqueue_g1(uint32_t M, const uint32_t* const* d_points, const uint32_t* con
                 int window_bits, hipStream_t s, MsmWork& w) {
   enqueue_curve<Fq>(M, d_points, d_scalars, ns, window_bits, s, w, 1);
 }
+uint64_t accumulate_threads_g1() { return accumulate_round_threads<Fq>(); }
 }  // namespace bg

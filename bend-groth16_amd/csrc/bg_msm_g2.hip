@@ -6,4 +6,5 @@ void enqueue_g2(uint32_t M, const uint32_t* const* d_points, const uint32_t* con
                 int window_bits, hipStream_t s, MsmWork& w) {
   enqueue_curve<Fq2>(M, d_points, d_scalars, ns, window_bits, s, w, 2);
 }
+uint64_t accumulate_threads_g2() { return accumulate_round_threads<Fq2>(); }
 }  // namespace bg
