@@ -125,8 +125,10 @@ def standalone_msm_pass(key: Path, hdr, reps: int) -> dict:
 
 def probe_peak() -> float:
     """Measured issue rate of v_mad_u64_u32 (16 independent chains per lane, four waves
-    per SIMD), best of two ~5 ms launches: the binding resource of the MSM kernels."""
-    return max(bendg.probe_rate(0, 65536)[0] for _ in range(2))
+    per SIMD), best of three ~20 ms launches after a warm-up: the binding resource of the
+    MSM kernels."""
+    bendg.probe_rate(0, 65536)
+    return max(bendg.probe_rate(0, 262144)[0] for _ in range(3))
 
 
 def traffic_from_profile() -> dict | None:

@@ -69,20 +69,69 @@ __device__ __forceinline__ void acc_shift(Acc& acc) {
   acc.hi = 0;
 }
 
+// 32-bit add / subtract steps with the carry or borrow in an SGPR pair (VOP3 forms), so a
+// 256-bit chain is one instruction per limb; the compiler otherwise builds each limb from
+// 64-bit adds and register moves (about 3x the instructions).  ALU-only assembly.
+__device__ __forceinline__ uint32_t add_co(uint32_t a, uint32_t b, uint64_t& co) {
+  uint32_t r;
+  asm("v_add_co_u32_e64 %0, %1, %2, %3" : "=v"(r), "=s"(co) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ uint32_t addc_co(uint32_t a, uint32_t b, uint64_t ci, uint64_t& co) {
+  uint32_t r;
+  asm("v_addc_co_u32_e64 %0, %1, %2, %3, %4" : "=v"(r), "=s"(co) : "v"(a), "v"(b), "s"(ci));
+  return r;
+}
+__device__ __forceinline__ uint32_t sub_co(uint32_t a, uint32_t b, uint64_t& bo) {
+  uint32_t r;
+  asm("v_sub_co_u32_e64 %0, %1, %2, %3" : "=v"(r), "=s"(bo) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ uint32_t subb_co(uint32_t a, uint32_t b, uint64_t bi, uint64_t& bo) {
+  uint32_t r;
+  asm("v_subb_co_u32_e64 %0, %1, %2, %3, %4" : "=v"(r), "=s"(bo) : "v"(a), "v"(b), "s"(bi));
+  return r;
+}
+// same with a wave-uniform constant as the second operand: an SGPR where the step has no
+// carry in, a VGPR where it has one (one scalar operand per VOP3 instruction)
+__device__ __forceinline__ uint32_t add_co_k(uint32_t a, uint32_t k, uint64_t& co) {
+  uint32_t r;
+  asm("v_add_co_u32_e64 %0, %1, %2, %3" : "=v"(r), "=s"(co) : "v"(a), "s"(k));
+  return r;
+}
+__device__ __forceinline__ uint32_t addc_co_k(uint32_t a, uint32_t k, uint64_t ci, uint64_t& co) {
+  uint32_t r;
+  asm("v_addc_co_u32_e64 %0, %1, %2, %3, %4" : "=v"(r), "=s"(co) : "v"(a), "v"(k), "s"(ci));
+  return r;
+}
+__device__ __forceinline__ uint32_t sub_co_k(uint32_t a, uint32_t k, uint64_t& bo) {
+  uint32_t r;
+  asm("v_sub_co_u32_e64 %0, %1, %2, %3" : "=v"(r), "=s"(bo) : "v"(a), "s"(k));
+  return r;
+}
+__device__ __forceinline__ uint32_t subb_co_k(uint32_t a, uint32_t k, uint64_t bi, uint64_t& bo) {
+  uint32_t r;
+  asm("v_subb_co_u32_e64 %0, %1, %2, %3, %4" : "=v"(r), "=s"(bo) : "v"(a), "v"(k), "s"(bi));
+  return r;
+}
+// m ? y : x per lane
+__device__ __forceinline__ uint32_t select(uint64_t m, uint32_t x, uint32_t y) {
+  uint32_t r;
+  asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(y), "s"(m));
+  return r;
+}
+
 // r = a - p if a >= p else a (a < 2p)
 template <class P>
 __device__ __forceinline__ void reduce_once(uint32_t* r, const uint32_t* a) {
   uint32_t d[8];
-  uint32_t br = 0;
+  uint64_t b;
+  d[0] = sub_co_k(a[0], P::p[0], b);
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    uint64_t t = (uint64_t)a[i] - P::p[i] - br;
-    d[i] = (uint32_t)t;
-    br = (uint32_t)(t >> 63);
-  }
-  // br == 1 -> a < p: keep a
+  for (int i = 1; i < 8; i++) d[i] = subb_co_k(a[i], P::p[i], b, b);
+  // borrow out -> a < p: keep a
 #pragma unroll
-  for (int i = 0; i < 8; i++) r[i] = br ? a[i] : d[i];
+  for (int i = 0; i < 8; i++) r[i] = select(b, d[i], a[i]);
 }
 
 // ------------------------------------------------------------------ field ops
@@ -118,37 +167,28 @@ struct Field {
 
   __device__ __forceinline__ static Fp add(const Fp& a, const Fp& b) {
     uint32_t s[8];
-    uint32_t c = 0;
+    uint64_t c;
+    s[0] = add_co(a.v[0], b.v[0], c);
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      uint64_t t = (uint64_t)a.v[i] + b.v[i] + c;
-      s[i] = (uint32_t)t;
-      c = (uint32_t)(t >> 32);
-    }
+    for (int i = 1; i < 8; i++) s[i] = addc_co(a.v[i], b.v[i], c, c);
     Fp r;
     reduce_once<P>(r.v, s);  // a + b < 2p < 2^255: no carry out of word 7
     return r;
   }
 
   __device__ __forceinline__ static Fp sub(const Fp& a, const Fp& b) {
-    uint32_t d[8];
-    uint32_t br = 0;
+    uint32_t d[8], t[8];
+    uint64_t br, c;
+    d[0] = sub_co(a.v[0], b.v[0], br);
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      uint64_t t = (uint64_t)a.v[i] - b.v[i] - br;
-      d[i] = (uint32_t)t;
-      br = (uint32_t)(t >> 63);
-    }
+    for (int i = 1; i < 8; i++) d[i] = subb_co(a.v[i], b.v[i], br, br);
     // borrow -> add p back
-    uint32_t mask = 0u - br;
-    Fp r;
-    uint32_t c = 0;
+    t[0] = add_co_k(d[0], P::p[0], c);
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      uint64_t t = (uint64_t)d[i] + (P::p[i] & mask) + c;
-      r.v[i] = (uint32_t)t;
-      c = (uint32_t)(t >> 32);
-    }
+    for (int i = 1; i < 8; i++) t[i] = addc_co_k(d[i], P::p[i], c, c);
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) r.v[i] = select(br, d[i], t[i]);
     return r;
   }
 
