@@ -169,6 +169,13 @@ struct Curve {
     return {F::mul(F::mul(p.X, p.ZZZ), t), F::mul(F::mul(p.Y, p.ZZ), t)};
   }
 
+  // the same, fully inlined (no device calls)
+  __device__ __forceinline__ static A to_affine_inline(const P& p) {
+    if (is_inf(p)) return {F::zero(), F::zero()};
+    E t = F::inv_inline(F::mul(p.ZZ, p.ZZZ));
+    return {F::mul(F::mul(p.X, p.ZZZ), t), F::mul(F::mul(p.Y, p.ZZ), t)};
+  }
+
   // memory layout of an XYZZ point: X, Y, ZZ, ZZZ (4 elements)
   static constexpr int elem_words = Ld<E>::words;
   static constexpr int xyzz_words = 4 * elem_words;

@@ -253,6 +253,22 @@ struct Field {
     return pow(a, e);
   }
 
+  // Fermat inverse a^(p-2) fully inlined (uniform loop over the exponent's bits, no
+  // device calls), for kernels that must stay call-free; inv(0) = 0
+  __device__ __forceinline__ static Fp inv_inline(const Fp& a) {
+    Fp acc = one();
+#pragma unroll
+    for (int wi = 7; wi >= 0; wi--) {
+      const uint32_t w = wi == 0 ? P::p[0] - 2u : P::p[wi];  // p - 2 (p[0] > 2: no borrow)
+#pragma unroll 1
+      for (int b = 31; b >= 0; b--) {
+        acc = sqr(acc);
+        if ((w >> b) & 1u) acc = mul(acc, a);
+      }
+    }
+    return acc;
+  }
+
   // canonical -> Montgomery (x * R^2 * R^-1)
   __device__ __forceinline__ static Fp to_mont(const Fp& a) {
     Fp r2;
@@ -306,6 +322,11 @@ struct Fq2 {
     Fp c1 = Fq::mul(Fq::dbl(a.c0), a.c1);
     __builtin_amdgcn_sched_barrier(0);
     return {c0, c1};
+  }
+  __device__ __forceinline__ static Fp2 inv_inline(const Fp2& a) {
+    Fp n = Fq::add(Fq::sqr(a.c0), Fq::sqr(a.c1));
+    Fp ni = Fq::inv_inline(n);
+    return {Fq::mul(a.c0, ni), Fq::neg(Fq::mul(a.c1, ni))};
   }
   __device__ __noinline__ static Fp2 inv(const Fp2& a) {
     Fp n = Fq::add(Fq::sqr(a.c0), Fq::sqr(a.c1));

@@ -27,6 +27,38 @@ uint64_t accumulate_threads_per_round(int group) {
   return group == 1 ? accumulate_threads_g1() : accumulate_threads_g2();
 }
 
+// reduction shape and accumulation slicing, given c, W, Wd, M, n
+static void finish_plan(MsmPlan& p, int group) {
+  p.B = 1u << (p.c - 1);
+  p.kspan = (uint32_t)p.W * p.B;
+  p.nb = p.M * p.kspan;
+  // buckets per level-1 group: 2, 4 or 8 (the four-role level-1 sums need m <= 8)
+  uint32_t mm = (uint32_t)tune_env("BG16_MSM_M");
+  mm = mm >= 8 || mm == 0 ? 8 : (mm >= 4 ? 4 : 2);
+  p.m = std::min<uint32_t>(mm, p.B);
+  p.G = p.B / p.m;
+  uint32_t kmax = group == 1 ? 256 : 128;
+  uint32_t kk = (uint32_t)tune_env("BG16_MSM_K");
+  if (kk) kmax = std::min(kmax, kk);
+  p.K = std::min<uint32_t>(kmax, p.G);
+  p.H = p.G / p.K;
+  p.entries = (uint64_t)p.Wd * p.n;
+  // entries per accumulation thread: about 20, rounded so that the launch fills a whole
+  // number of device rounds (threads resident at once), which leaves no tail of a few
+  // waves; longer slices also leave fewer partial runs to merge
+  uint32_t lenv = (uint32_t)tune_env("BG16_MSM_L");
+  uint64_t L;
+  if (lenv) {
+    L = lenv;
+  } else {
+    const uint64_t cap = accumulate_threads_per_round(group);
+    const uint64_t k = std::max<uint64_t>(1, p.entries / (20 * cap));
+    L = (p.entries + k * cap - 1) / (k * cap);
+  }
+  p.L = (uint32_t)std::max<uint64_t>(2, std::min<uint64_t>(64, L));
+  p.acc_threads = (uint32_t)((p.entries + p.L - 1) / p.L);
+}
+
 MsmPlan msm_plan_batch(const uint64_t* ns, uint32_t M, int group, int window_bits) {
   MsmPlan p;
   if (window_bits <= 0) window_bits = tune_env(group == 1 ? "BG16_MSM_C_G1" : "BG16_MSM_C_G2");
@@ -48,35 +80,39 @@ MsmPlan msm_plan_batch(const uint64_t* ns, uint32_t M, int group, int window_bit
     }
   }
   p.c = c;
-  p.W = (255 + c - 1) / c;
-  p.B = 1u << (c - 1);
-  p.kspan = (uint32_t)p.W * p.B;
-  p.nb = M * p.kspan;
-  // buckets per level-1 group: 2, 4 or 8 (the four-lane level-1 sums need m <= 8)
-  uint32_t mm = (uint32_t)tune_env("BG16_MSM_M");
-  mm = mm >= 8 || mm == 0 ? 8 : (mm >= 4 ? 4 : 2);
-  p.m = std::min<uint32_t>(mm, p.B);
-  p.G = p.B / p.m;
-  uint32_t kmax = group == 1 ? 256 : 128;
-  uint32_t kk = (uint32_t)tune_env("BG16_MSM_K");
-  if (kk) kmax = std::min(kmax, kk);
-  p.K = std::min<uint32_t>(kmax, p.G);
-  p.H = p.G / p.K;
-  p.entries = (uint64_t)p.W * n;
-  // entries per accumulation thread: about 20, rounded so that the launch fills a whole
-  // number of device rounds (threads resident at once), which leaves no tail of a few
-  // waves; longer slices also leave fewer partial runs to merge
-  uint32_t lenv = (uint32_t)tune_env("BG16_MSM_L");
-  uint64_t L;
-  if (lenv) {
-    L = lenv;
-  } else {
-    const uint64_t cap = accumulate_threads_per_round(group);
-    const uint64_t k = std::max<uint64_t>(1, p.entries / (20 * cap));
-    L = (p.entries + k * cap - 1) / (k * cap);
+  p.W = p.Wd = (255 + c - 1) / c;
+  finish_plan(p, group);
+  return p;
+}
+
+int fixed_window_bits(const uint64_t* ns, uint32_t M, int group) {
+  int env = tune_env(group == 1 ? "BG16_FIXED_C_G1" : "BG16_FIXED_C_G2");
+  if (env > 0) return env;
+  uint64_t n = 0;
+  for (uint32_t i = 0; i < M; i++) n += ns[i];
+  // every window shares one set of buckets: minimise n*Wd*madd + M*2^(c-1)*2*add
+  double best = 1e300;
+  int c = 16;
+  for (int cc = 8; cc <= 20; cc++) {
+    int Wd = (255 + cc - 1) / cc;
+    double cost = (double)n * Wd * 10.0 + (double)M * (double)(1u << (cc - 1)) * 28.0;
+    if (cost < best) {
+      best = cost;
+      c = cc;
+    }
   }
-  p.L = (uint32_t)std::max<uint64_t>(2, std::min<uint64_t>(64, L));
-  p.acc_threads = (uint32_t)((p.entries + p.L - 1) / p.L);
+  return c;
+}
+
+MsmPlan msm_plan_fixed(const uint64_t* ns, uint32_t M, int group, int c) {
+  MsmPlan p;
+  p.M = M;
+  for (uint32_t i = 0; i < M; i++) p.n += ns[i];
+  p.c = c;
+  p.fixed = true;
+  p.W = 1;
+  p.Wd = (255 + c - 1) / c;
+  finish_plan(p, group);
   return p;
 }
 
@@ -96,7 +132,7 @@ uint64_t msm_work_bytes(const MsmPlan& p, int group) {
                + 2 * ngroups * xb + 2 * nout * xb + (1u << 16)
                + 32 * p.n                         // reduced scalars
                + 4 * (ngroups + 256) * xb          // level-1 role sums
-               + 4ull * p.W * p.B * (32 * p.M + p.n / 16384 + 1);  // chunk histograms (bound)
+               + 4ull * p.Wd * p.B * (33 * p.M + 2 * p.n / (p.B > 1 ? p.B : 1) + 1);  // chunk histograms (bound)
   return b + b / 8;  // grow-only buffers keep 1/8 slack
 }
 
@@ -177,7 +213,7 @@ static constexpr uint32_t HIST_MAX = 16384;
 static constexpr int COUNT_T = 1024;
 
 __global__ __launch_bounds__(COUNT_T) void k_count(const uint32_t* __restrict__ sc, uint32_t n, int c,
-                                                   uint32_t chunk, uint32_t nchunks, uint32_t key_base,
+                                                   uint32_t chunk, uint32_t nchunks, uint32_t key_base, int fixed,
                                                    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
                                                    uint32_t* __restrict__ ranks, uint32_t* __restrict__ hist_out) {
   __shared__ uint32_t hist[HIST_MAX];
@@ -210,8 +246,10 @@ __global__ __launch_bounds__(COUNT_T) void k_count(const uint32_t* __restrict__ 
       uint32_t bk = (neg ? full - v : v) - 1;
       owner = bk - lo_b < Bp;
       if (owner) {
-        key = key_base + w * B + bk;
-        val = (uint32_t)i | (neg ? 0x80000000u : 0u);
+        // windowed bases: bucket (w, bk), point i; fixed-base tables: bucket bk shared by
+        // all windows, point w * n + i of the table
+        key = key_base + (fixed ? 0u : w * B) + bk;
+        val = (fixed ? w * n + (uint32_t)i : (uint32_t)i) | (neg ? 0x80000000u : 0u);
         r = atomicAdd(&hist[bk - lo_b], 1u);
       }
     }
@@ -315,18 +353,26 @@ __global__ void k_zero_u32(uint32_t* __restrict__ out, uint64_t n) {
 
 // sorted[offsets[key] + chunk base + rank] = (key, val), one MSM's entries
 __global__ void k_scatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                          const uint32_t* __restrict__ ranks, uint32_t n, uint32_t W, uint32_t B, uint32_t chunk,
-                          uint32_t nchunks, uint32_t key_base, const uint32_t* __restrict__ hist,
+                          const uint32_t* __restrict__ ranks, uint32_t n, uint32_t Wd, uint32_t B, uint32_t chunk,
+                          uint32_t nchunks, uint32_t key_base, int fixed, const uint32_t* __restrict__ hist,
                           const uint32_t* __restrict__ offsets, uint2* __restrict__ sorted) {
   uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (uint64_t)W * n) return;
+  if (e >= (uint64_t)Wd * n) return;
   uint32_t key = keys[e];
   if (key == NOKEY) return;
   uint32_t val = vals[e];
-  uint32_t kb = key - key_base;  // w * B + bucket
-  uint32_t w = kb / B, b = kb % B;
-  uint32_t ck = (val & 0x7FFFFFFFu) / chunk;
-  uint32_t base = hist[((uint64_t)w * nchunks + ck) * B + b];
+  uint32_t kb = key - key_base, idx = val & 0x7FFFFFFFu;
+  uint32_t w, b, i;
+  if (fixed) {
+    w = idx / n;
+    i = idx % n;
+    b = kb;
+  } else {
+    w = kb / B;
+    b = kb % B;
+    i = idx;
+  }
+  uint32_t base = hist[((uint64_t)w * nchunks + i / chunk) * B + b];
   sorted[offsets[key] + base + ranks[e]] = make_uint2(key, val);
 }
 
@@ -364,9 +410,10 @@ void launch_zero_u32(uint32_t* p, uint64_t n, hipStream_t s) {
   BG_HIP(hipGetLastError());
 }
 
-// chunk of scalars per counting block: at least 16384, at most 32 chunks per MSM
-static uint32_t count_chunk(uint64_t n) {
-  uint64_t c = (n + 31) / 32;
+// chunk of scalars per counting block: at least 16384, at most 32 chunks per MSM, and at
+// least half the bucket count (bounds the chunk histograms for wide windows)
+static uint32_t count_chunk(uint64_t n, uint32_t B) {
+  uint64_t c = std::max<uint64_t>((n + 31) / 32, B / 2);
   c = (c + 1023) / 1024 * 1024;
   return (uint32_t)std::max<uint64_t>(16384, c);
 }
@@ -374,7 +421,10 @@ static uint32_t count_chunk(uint64_t n) {
 uint64_t msm_hist_words(const MsmPlan& p, const uint64_t* ns) {
   uint64_t t = 0;
   for (uint32_t i = 0; i < p.M; i++)
-    if (ns[i]) t += (uint64_t)p.W * ((ns[i] + count_chunk(ns[i]) - 1) / count_chunk(ns[i])) * p.B;
+    if (ns[i]) {
+      const uint32_t ch = count_chunk(ns[i], p.B);
+      t += (uint64_t)p.Wd * ((ns[i] + ch - 1) / ch) * p.B;
+    }
   return std::max<uint64_t>(t, 1);
 }
 
@@ -385,6 +435,7 @@ void msm_sort(const uint32_t* const* d_scalars, const uint32_t* const* d_points,
               uint32_t* scal, uint32_t* hist) {
   uint32_t nblk = msm_scan_blocks(p.nb);
   const uint32_t parts = p.B > HIST_MAX ? p.B / HIST_MAX : 1;
+  const int fixed = p.fixed ? 1 : 0;
   {
     ScopedKTime kt(group == 1 ? "msm_g1_digits" : "msm_g2_digits", s);
     k_zero_u32<<<blocks_for(p.nb + 1, 256), 256, 0, s>>>(counts, (uint64_t)p.nb + 1);
@@ -393,16 +444,21 @@ void msm_sort(const uint32_t* const* d_scalars, const uint32_t* const* d_points,
     for (uint32_t i = 0; i < p.M; i++) {
       const uint64_t n = ns[i];
       if (n) {
-        const uint32_t ch = count_chunk(n), nch = (uint32_t)((n + ch - 1) / ch);
+        const uint32_t ch = count_chunk(n, p.B), nch = (uint32_t)((n + ch - 1) / ch);
         k_prep_scalars<<<blocks_for(n, 256), 256, 0, s>>>(d_scalars[i], d_points[i], group == 1 ? 16 : 32,
                                                            (uint32_t)n, scal + soff);
-        k_count<<<dim3(nch, p.W, parts), COUNT_T, 0, s>>>(scal + soff, (uint32_t)n, p.c, ch, nch, i * p.kspan,
-                                                          keys + off, vals + off, ranks + off, hist + hoff);
-        k_column_scan<<<blocks_for((uint64_t)p.W * p.B, 256), 256, 0, s>>>(hist + hoff, p.W, p.B, nch,
-                                                                           counts + (uint64_t)i * p.kspan);
-        hoff += (uint64_t)p.W * nch * p.B;
+        k_count<<<dim3(nch, p.Wd, parts), COUNT_T, 0, s>>>(scal + soff, (uint32_t)n, p.c, ch, nch, i * p.kspan,
+                                                           fixed, keys + off, vals + off, ranks + off, hist + hoff);
+        // a bucket's column runs over every (window, chunk) histogram that feeds it
+        if (fixed)
+          k_column_scan<<<blocks_for(p.B, 256), 256, 0, s>>>(hist + hoff, 1, p.B, (uint32_t)p.Wd * nch,
+                                                             counts + (uint64_t)i * p.kspan);
+        else
+          k_column_scan<<<blocks_for((uint64_t)p.W * p.B, 256), 256, 0, s>>>(hist + hoff, p.W, p.B, nch,
+                                                                             counts + (uint64_t)i * p.kspan);
+        hoff += (uint64_t)p.Wd * nch * p.B;
       }
-      off += (uint64_t)p.W * n;
+      off += (uint64_t)p.Wd * n;
       soff += 8 * n;
     }
     BG_HIP(hipGetLastError());
@@ -416,13 +472,13 @@ void msm_sort(const uint32_t* const* d_scalars, const uint32_t* const* d_points,
     for (uint32_t i = 0; i < p.M; i++) {
       const uint64_t n = ns[i];
       if (n) {
-        const uint32_t ch = count_chunk(n), nch = (uint32_t)((n + ch - 1) / ch);
-        k_scatter<<<blocks_for((uint64_t)p.W * n, 256), 256, 0, s>>>(keys + off, vals + off, ranks + off,
-                                                                     (uint32_t)n, p.W, p.B, ch, nch, i * p.kspan,
-                                                                     hist + hoff, offsets, sorted);
-        hoff += (uint64_t)p.W * nch * p.B;
+        const uint32_t ch = count_chunk(n, p.B), nch = (uint32_t)((n + ch - 1) / ch);
+        k_scatter<<<blocks_for((uint64_t)p.Wd * n, 256), 256, 0, s>>>(keys + off, vals + off, ranks + off,
+                                                                      (uint32_t)n, p.Wd, p.B, ch, nch, i * p.kspan,
+                                                                      fixed, hist + hoff, offsets, sorted);
+        hoff += (uint64_t)p.Wd * nch * p.B;
       }
-      off += (uint64_t)p.W * n;
+      off += (uint64_t)p.Wd * n;
     }
     BG_HIP(hipGetLastError());
   }
@@ -431,9 +487,24 @@ void msm_sort(const uint32_t* const* d_scalars, const uint32_t* const* d_points,
 void msm_enqueue_batch(int group, uint32_t M, const uint32_t* const* d_points, const uint32_t* const* d_scalars,
                        const uint64_t* ns, int window_bits, hipStream_t s, MsmWork& w) {
   if (group == 1)
-    enqueue_g1(M, d_points, d_scalars, ns, window_bits, s, w);
+    enqueue_g1(M, d_points, d_scalars, ns, window_bits, s, w, false);
   else
-    enqueue_g2(M, d_points, d_scalars, ns, window_bits, s, w);
+    enqueue_g2(M, d_points, d_scalars, ns, window_bits, s, w, false);
+}
+
+void msm_enqueue_fixed(int group, uint32_t M, const uint32_t* const* d_tables, const uint32_t* const* d_scalars,
+                       const uint64_t* ns, int c, hipStream_t s, MsmWork& w) {
+  if (group == 1)
+    enqueue_g1(M, d_tables, d_scalars, ns, c, s, w, true);
+  else
+    enqueue_g2(M, d_tables, d_scalars, ns, c, s, w, true);
+}
+
+void msm_build_fixed_table(int group, const uint32_t* d_points, uint64_t n, int c, uint32_t* d_out, hipStream_t s) {
+  if (group == 1)
+    fixed_table_g1(d_points, n, c, d_out, s);
+  else
+    fixed_table_g2(d_points, n, c, d_out, s);
 }
 
 void msm_enqueue(int group, const uint32_t* d_points, const uint32_t* d_scalars, uint64_t n, int window_bits,

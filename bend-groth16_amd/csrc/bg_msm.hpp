@@ -19,7 +19,10 @@ namespace bg {
 // runs once over all MSMs.
 struct MsmPlan {
   int c = 0;          // window width (bits)
-  int W = 0;          // windows per MSM, W*c >= 255
+  int W = 0;          // bucket windows per MSM (1 with fixed-base tables)
+  int Wd = 0;         // digit windows per scalar, Wd*c >= 255 (= W without tables)
+  bool fixed = false; // bases are fixed-base tables: entry (w, i) uses 2^(wc) P_i, every
+                      // window adds into the same buckets
   uint32_t M = 1;     // MSMs in the batch
   uint32_t B = 0;     // buckets per window, 2^(c-1)
   uint32_t kspan = 0; // buckets per MSM, W*B
@@ -30,12 +33,16 @@ struct MsmPlan {
   uint32_t H = 0;     // level-2 outputs per window
   uint32_t L = 0;     // sorted entries per accumulation thread
   uint64_t n = 0;     // total points
-  uint64_t entries = 0;  // W * total points
+  uint64_t entries = 0;  // Wd * total points
   uint32_t acc_threads = 0;
 };
 
 MsmPlan msm_plan(uint64_t n, int group, int window_bits);
 MsmPlan msm_plan_batch(const uint64_t* ns, uint32_t M, int group, int window_bits);
+// Plan over fixed-base tables of window width c (table w of MSM i at w * n_i).
+MsmPlan msm_plan_fixed(const uint64_t* ns, uint32_t M, int group, int c);
+// Window width for fixed-base tables of a batch of MSMs of these sizes.
+int fixed_window_bits(const uint64_t* ns, uint32_t M, int group);
 
 // Work buffers of one in-flight MSM (reused across calls).
 struct MsmWork {
@@ -84,6 +91,12 @@ void msm_enqueue(int group, const uint32_t* d_points, const uint32_t* d_scalars,
 // Batch of M MSMs of one group: points[i] / scalars[i] / ns[i] per MSM (device).
 void msm_enqueue_batch(int group, uint32_t M, const uint32_t* const* d_points, const uint32_t* const* d_scalars,
                        const uint64_t* ns, int window_bits, hipStream_t s, MsmWork& w);
+// Batch over fixed-base tables (built by msm_build_fixed_table with the same c): d_tables[i]
+// holds Wd * ns[i] affine points.
+void msm_enqueue_fixed(int group, uint32_t M, const uint32_t* const* d_tables, const uint32_t* const* d_scalars,
+                       const uint64_t* ns, int c, hipStream_t s, MsmWork& w);
+// out[w * n + i] = 2^(w c) points[i] (affine), w < ceil(255 / c); out holds that many points.
+void msm_build_fixed_table(int group, const uint32_t* d_points, uint64_t n, int c, uint32_t* d_out, hipStream_t s);
 
 // Waits for the MSM(s) and finishes them on the host: out = affine results, one per MSM
 // in the batch (packed words: G1 16, G2 32 each).  xyzz_out (optional) receives the XYZZ
@@ -103,8 +116,10 @@ void msm_sort(const uint32_t* const* d_scalars, const uint32_t* const* d_points,
               uint32_t* counts, uint32_t* offsets, uint32_t* bsum, uint2* sorted, uint32_t* fix, uint64_t fix_words,
               uint32_t* scal, uint32_t* hist);
 void enqueue_g1(uint32_t M, const uint32_t* const* d_points, const uint32_t* const* d_scalars, const uint64_t* ns,
-                int window_bits, hipStream_t s, MsmWork& w);
+                int window_bits, hipStream_t s, MsmWork& w, bool fixed);
 void enqueue_g2(uint32_t M, const uint32_t* const* d_points, const uint32_t* const* d_scalars, const uint64_t* ns,
-                int window_bits, hipStream_t s, MsmWork& w);
+                int window_bits, hipStream_t s, MsmWork& w, bool fixed);
+void fixed_table_g1(const uint32_t* d_points, uint64_t n, int c, uint32_t* d_out, hipStream_t s);
+void fixed_table_g2(const uint32_t* d_points, uint64_t n, int c, uint32_t* d_out, hipStream_t s);
 
 }  // namespace bg

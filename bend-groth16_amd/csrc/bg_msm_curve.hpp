@@ -403,14 +403,48 @@ uint64_t accumulate_round_threads() {
   return cached;
 }
 
+// Fixed-base table: out[w * n + i] = 2^(w c) P_i (affine), one thread per point walking
+// the windows (c doublings and one inversion per window, all inlined); the point at
+// infinity (all-zero affine) stays all-zero in every window.
+template <class F>
+__global__ __launch_bounds__(256) void k_fixed_table(const uint32_t* __restrict__ pts, uint64_t n, int c, int Wd,
+                                                     uint32_t* __restrict__ out) {
+  using C = Curve<F>;
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t aw = C::aff_words;
+  typename C::A a = C::load_aff(pts + aw * i);
+  C::store_aff(out + aw * i, a);
+  const bool inf = C::aff_is_inf(a);
+  typename C::P acc = C::from_affine(a);
+  for (int w = 1; w < Wd; w++) {
+    for (int k = 0; k < c; k++) acc = C::dbl(acc);
+    typename C::A r = inf ? a : C::to_affine_inline(acc);
+    C::store_aff(out + aw * ((uint64_t)w * n + i), r);
+  }
+}
+
+template <class F>
+void fixed_table_curve(const uint32_t* d_points, uint64_t n, int c, uint32_t* d_out, hipStream_t s) {
+  if (!n) return;
+  const int Wd = (255 + c - 1) / c;
+  k_fixed_table<F><<<blocks_of(n, 256), 256, 0, s>>>(d_points, n, c, Wd, d_out);
+  BG_HIP(hipGetLastError());
+}
+
 template <class F>
 void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* const* d_scalars, const uint64_t* ns,
-                   int window_bits, hipStream_t s, MsmWork& w, int group) {
+                   int window_bits, hipStream_t s, MsmWork& w, int group, bool fixed) {
   using C = Curve<F>;
   BG_REQUIRE(M >= 1, BG_ERR_ARG, "msm: empty batch");
   for (uint32_t i = 0; i < M; i++)
     BG_REQUIRE(ns[i] < (1ull << 31), BG_ERR_ARG, "msm: n = %llu exceeds 2^31 - 1", (unsigned long long)ns[i]);
-  MsmPlan p = msm_plan_batch(ns, M, group, window_bits);
+  if (fixed) BG_REQUIRE(window_bits >= 2 && window_bits <= 22, BG_ERR_ARG, "msm: fixed-base tables need c");
+  MsmPlan p = fixed ? msm_plan_fixed(ns, M, group, window_bits) : msm_plan_batch(ns, M, group, window_bits);
+  if (fixed)
+    for (uint32_t i = 0; i < M; i++)
+      BG_REQUIRE((uint64_t)p.Wd * ns[i] < (1ull << 31), BG_ERR_ARG, "msm: fixed-base table of %llu points too large",
+                 (unsigned long long)ns[i]);
   const uint64_t n = p.n;
   BG_REQUIRE(p.entries < (1ull << 32) && (uint64_t)M * p.kspan < (1ull << 31), BG_ERR_ARG,
              "msm: batch too large (%llu entries)", (unsigned long long)p.entries);

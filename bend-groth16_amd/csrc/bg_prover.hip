@@ -69,6 +69,11 @@ struct bg_pk {
   bg::DevArray pa, pb1, pb2, ph, pl;
   uint32_t g1[3 * 16];  // alpha, beta, delta (packed affine)
   uint32_t g2[2 * 32];  // beta2, delta2
+  // fixed-base tables: each query array holds Wd copies of its points, window w scaled by
+  // 2^(w c); c for the G1 batch (A, B1, L), for H and for B2.  fixed = false keeps the
+  // plain arrays (windowed MSMs).
+  bool fixed = false;
+  int c_g1 = 0, c_h = 0, c_b2 = 0;
   uint64_t dev_bytes = 0;  // resident key bytes
   // slot used by bg_prove (created on first use; one bg_prove call per key at a time)
   std::unique_ptr<bg::ProveSlot> own;
@@ -211,7 +216,30 @@ extern "C" int bg_pk_load(const char* dir, bg_pk** out) {
       upload_array(pr.second->cols, pr.first->cols.data(), 4ull * pr.first->nnz, total);
       upload_array(pr.second->coef, pr.first->coef.data(), 32ull * pr.first->nnz, total);
     }
-    for (auto& q : qs) upload_array(*q.dst, q.host.data(), q.host.size() * 4, total);
+    const char* fb = getenv("BG16_FIXED_BASE");
+    pk->fixed = !(fb && fb[0] == '0');
+    if (!pk->fixed) {
+      for (auto& q : qs) upload_array(*q.dst, q.host.data(), q.host.size() * 4, total);
+    } else {
+      // tables built on the device from the uploaded points (window 0 is the point itself)
+      const uint64_t g1n[3] = {na, nbq, nl};
+      pk->c_g1 = fixed_window_bits(g1n, 3, 1);
+      pk->c_h = fixed_window_bits(&qs[3].n, 1, 1);
+      pk->c_b2 = fixed_window_bits(&qs[2].n, 1, 2);
+      const int cs[5] = {pk->c_g1, pk->c_g1, pk->c_b2, pk->c_h, pk->c_g1};
+      for (int k = 0; k < 5; k++) {
+        auto& q = qs[k];
+        const int group = k == 2 ? 2 : 1;
+        const uint64_t wd = (255 + cs[k] - 1) / cs[k];
+        DevArray plain;
+        upload_array(plain, q.host.data(), q.host.size() * 4, total);
+        total -= q.host.size() * 4;
+        q.dst->alloc(std::max<uint64_t>(wd * q.host.size() * 4, 4));
+        total += wd * q.host.size() * 4;
+        msm_build_fixed_table(group, plain.as<uint32_t>(), q.n, cs[k], q.dst->as<uint32_t>(), c.stream);
+        BG_HIP(hipStreamSynchronize(c.stream));
+      }
+    }
     pk->dev_bytes = total;
     (void)ntt_domain((int)lg);  // twiddle / scale tables resident with the key
     *out = pk.release();
@@ -290,9 +318,15 @@ uint64_t prove_bytes_estimate(const bg_pk* pk) {
   const uint64_t nz = (uint64_t)ni + nw, d = 1ull << lg;
   uint64_t g1n[3] = {na, nbq, nl};
   uint64_t t = 128 * nz + 160 * d;
-  t += msm_work_bytes(msm_plan_batch(g1n, 3, 1, 0), 1);
-  t += msm_work_bytes(msm_plan(nh, 1, 0), 1);
-  t += msm_work_bytes(msm_plan(nbq, 2, 0), 2);
+  if (pk->fixed) {
+    t += msm_work_bytes(msm_plan_fixed(g1n, 3, 1, pk->c_g1), 1);
+    t += msm_work_bytes(msm_plan_fixed(&nh, 1, 1, pk->c_h), 1);
+    t += msm_work_bytes(msm_plan_fixed(&nbq, 1, 2, pk->c_b2), 2);
+  } else {
+    t += msm_work_bytes(msm_plan_batch(g1n, 3, 1, 0), 1);
+    t += msm_work_bytes(msm_plan(nh, 1, 0), 1);
+    t += msm_work_bytes(msm_plan(nbq, 2, 0), 2);
+  }
   return t;
 }
 
@@ -380,12 +414,21 @@ void prove_on(bg_pk* pk, ProveSlot& sl, const uint32_t* witness, const uint32_t*
     };
     auto run_b2 = [&] {
       BG_HIP(hipEventRecord(ev[9], s3));
-      msm_enqueue(2, pk->pb2.as<uint32_t>(), zc, nbq, 0, s3, sl.work[2]);
+      if (pk->fixed) {
+        const uint32_t* tb = pk->pb2.as<uint32_t>();
+        const uint64_t nn = nbq;
+        msm_enqueue_fixed(2, 1, &tb, &zc, &nn, pk->c_b2, s3, sl.work[2]);
+      } else {
+        msm_enqueue(2, pk->pb2.as<uint32_t>(), zc, nbq, 0, s3, sl.work[2]);
+      }
       BG_HIP(hipEventRecord(ev[10], s3));
     };
     auto run_g1_batch3 = [&] {
       BG_HIP(hipEventRecord(ev[5], s1));
-      msm_enqueue_batch(1, 3, g1p, g1s, g1n, 0, s1, sl.work[0]);
+      if (pk->fixed)
+        msm_enqueue_fixed(1, 3, g1p, g1s, g1n, pk->c_g1, s1, sl.work[0]);
+      else
+        msm_enqueue_batch(1, 3, g1p, g1s, g1n, 0, s1, sl.work[0]);
       BG_HIP(hipEventRecord(ev[6], s1));
     };
     if (sched == 4) {
@@ -407,10 +450,26 @@ void prove_on(bg_pk* pk, ProveSlot& sl, const uint32_t* witness, const uint32_t*
       }
     }
     if (sched != 1) {
-      msm_enqueue(1, pk->ph.as<uint32_t>(), sl.h.as<uint32_t>(), nh, 0, s0, sl.work[1]);
+      if (pk->fixed) {
+        const uint32_t* tb = pk->ph.as<uint32_t>();
+        const uint32_t* hs = sl.h.as<uint32_t>();
+        const uint64_t nn = nh;
+        msm_enqueue_fixed(1, 1, &tb, &hs, &nn, pk->c_h, s0, sl.work[1]);
+      } else {
+        msm_enqueue(1, pk->ph.as<uint32_t>(), sl.h.as<uint32_t>(), nh, 0, s0, sl.work[1]);
+      }
     } else {
       BG_HIP(hipEventRecord(ev[5], s0));
-      msm_enqueue_batch(1, 4, g1p, g1s, g1n, 0, s0, sl.work[0]);
+      if (pk->fixed) {
+        // schedule 1 batches H with A, B1, L: its table has its own width, so H runs alone
+        msm_enqueue_fixed(1, 3, g1p, g1s, g1n, pk->c_g1, s0, sl.work[0]);
+        const uint32_t* tb = pk->ph.as<uint32_t>();
+        const uint32_t* hs = sl.h.as<uint32_t>();
+        const uint64_t nn = nh;
+        msm_enqueue_fixed(1, 1, &tb, &hs, &nn, pk->c_h, s0, sl.work[1]);
+      } else {
+        msm_enqueue_batch(1, 4, g1p, g1s, g1n, 0, s0, sl.work[0]);
+      }
       BG_HIP(hipEventRecord(ev[6], s0));
     }
     BG_HIP(hipEventRecord(ev[11], s0));
@@ -449,7 +508,7 @@ void prove_on(bg_pk* pk, ProveSlot& sl, const uint32_t* witness, const uint32_t*
     msm_finish(sl.work[2], mB2);
     bgh::P2 B2 = bgh::load_affine<bgh::Fq2>(mB2).add(s_d2).add(beta2);
     bgh::store_affine(B2, proof + 16);
-    if (sched != 1) {
+    if (sched != 1 || pk->fixed) {
       msm_finish(sl.work[1], mHs);
       mH = mHs;
     }

@@ -99,3 +99,24 @@ def test_prove_large_closed_form(bg, tmp_path, lg):
     eA, eB, eC = points_of(closed_form(load(tmp_path)))
     assert np.array_equal(A, eA) and np.array_equal(B, eB) and np.array_equal(C, eC)
     pk.close()
+
+
+def test_prove_windowed_and_fixed_base_agree(bg, keys, monkeypatch):
+    """Keys load with fixed-base tables by default (2^(wc) P_i per window, one set of
+    buckets for all windows); BG16_FIXED_BASE=0 keeps the plain windowed MSMs.  Both give
+    the same proof, equal to the oracle's."""
+    d = keys[12]
+    z, rs = _files(d)
+    pk_fixed = bendg.ProvingKey(d)
+    monkeypatch.setenv("BG16_FIXED_BASE", "0")
+    pk_plain = bendg.ProvingKey(d)
+    monkeypatch.delenv("BG16_FIXED_BASE")
+    a, _ = pk_fixed.prove(z, rs)
+    b, _ = pk_plain.prove(z, rs)
+    assert np.array_equal(a, b)
+    assert pk_fixed.device_bytes() > pk_plain.device_bytes()
+    oA, oB, oC, _ = O.prove(load(d))
+    A, B, C = _split(bendg.pack16(a.reshape(-1, 16)).reshape(-1))
+    assert np.array_equal(A, oA) and np.array_equal(B, oB) and np.array_equal(C, oC)
+    pk_fixed.close()
+    pk_plain.close()
