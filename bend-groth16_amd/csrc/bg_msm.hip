@@ -206,19 +206,27 @@ __device__ __forceinline__ uint32_t window_carry(const uint32_t* s, const uint32
 
 // Counting sort of (bucket, point) entries without global atomics (radix-sort layout):
 // block (chunk, window, part) ranks its chunk's digits for one window in an LDS
-// histogram of up to HIST_MAX buckets, writes (key, index | sign, rank in block) for
+// histogram (see k_count), writes (key, index | sign, rank in block) for
 // each entry it owns and its histogram to hist[(window * nchunks + chunk) * B + bucket];
 // k_column_scan turns the histograms into per-chunk bucket bases and bucket totals.
-static constexpr uint32_t HIST_MAX = 16384;
+// LDS histogram of 64 KB: with chunks of at most 2^15 scalars the counters are 16-bit,
+// two to a word (32768 buckets per block; no counter reaches 2^16); larger chunks use
+// 32-bit counters (16384 buckets per block).  Wider windows split the bucket range over
+// blockIdx.z.  The rank word also carries the window in bits 24..31, so the scatter finds
+// the entry's chunk histogram without dividing.
+static constexpr uint32_t HIST_WORDS = 16384;
 static constexpr int COUNT_T = 1024;
+static constexpr int PACKED_CHUNK_LOG2 = 15;
 
+template <bool P16>
 __global__ __launch_bounds__(COUNT_T) void k_count(const uint32_t* __restrict__ sc, uint32_t n, int c,
                                                    uint32_t chunk, uint32_t nchunks, uint32_t key_base, int fixed,
                                                    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
                                                    uint32_t* __restrict__ ranks, uint32_t* __restrict__ hist_out) {
-  __shared__ uint32_t hist[HIST_MAX];
+  __shared__ uint32_t hist[HIST_WORDS];
+  constexpr uint32_t HMAX = P16 ? 2 * HIST_WORDS : HIST_WORDS;
   const uint32_t B = 1u << (c - 1);
-  const uint32_t Bp = B < HIST_MAX ? B : HIST_MAX;
+  const uint32_t Bp = B < HMAX ? B : HMAX;
   const uint32_t w = blockIdx.y;
   const uint32_t lo_b = blockIdx.z * Bp;
   const int nbits = (int)w * c;
@@ -227,7 +235,8 @@ __global__ __launch_bounds__(COUNT_T) void k_count(const uint32_t* __restrict__ 
     int pos = j * c + c - 1;
     if (pos < 256) pat[pos >> 5] |= 1u << (pos & 31);
   }
-  for (uint32_t b = threadIdx.x; b < Bp; b += COUNT_T) hist[b] = 0;
+  const uint32_t hw = P16 ? (Bp + 1) / 2 : Bp;
+  for (uint32_t b = threadIdx.x; b < hw; b += COUNT_T) hist[b] = 0;
   __syncthreads();
   const uint64_t i0 = (uint64_t)blockIdx.x * chunk;
   const uint64_t i1 = min(i0 + chunk, (uint64_t)n);
@@ -248,21 +257,28 @@ __global__ __launch_bounds__(COUNT_T) void k_count(const uint32_t* __restrict__ 
       if (owner) {
         // windowed bases: bucket (w, bk), point i; fixed-base tables: bucket bk shared by
         // all windows, point w * n + i of the table
+        const uint32_t lb = bk - lo_b;
         key = key_base + (fixed ? 0u : w * B) + bk;
         val = (fixed ? w * n + (uint32_t)i : (uint32_t)i) | (neg ? 0x80000000u : 0u);
-        r = atomicAdd(&hist[bk - lo_b], 1u);
+        if (P16) {
+          const uint32_t sh = 16 * (lb & 1u);
+          r = (atomicAdd(&hist[lb >> 1], 1u << sh) >> sh) & 0xFFFFu;
+        } else {
+          r = atomicAdd(&hist[lb], 1u);
+        }
       }
     }
     if (owner) {
       uint64_t e = (uint64_t)w * n + i;
       keys[e] = key;
       vals[e] = val;
-      ranks[e] = r;
+      ranks[e] = r | (w << 24);
     }
   }
   __syncthreads();
   uint32_t* ho = hist_out + ((uint64_t)w * nchunks + blockIdx.x) * B + lo_b;
-  for (uint32_t b = threadIdx.x; b < Bp; b += COUNT_T) ho[b] = hist[b];
+  for (uint32_t b = threadIdx.x; b < Bp; b += COUNT_T)
+    ho[b] = P16 ? (hist[b >> 1] >> (16 * (b & 1u))) & 0xFFFFu : hist[b];
 }
 
 // For every (window, bucket) of one MSM: exclusive prefix of the chunk histograms in
@@ -274,7 +290,18 @@ __global__ void k_column_scan(uint32_t* __restrict__ hist, uint32_t W, uint32_t 
   uint32_t w = (uint32_t)(t / B), b = (uint32_t)(t % B);
   uint32_t* col = hist + (uint64_t)w * nchunks * B + b;
   uint32_t run = 0;
-  for (uint32_t k = 0; k < nchunks; k++) {
+  uint32_t k = 0;
+  for (; k + 8 <= nchunks; k += 8) {  // eight loads in flight
+    uint32_t x[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) x[u] = col[(uint64_t)(k + u) * B];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      col[(uint64_t)(k + u) * B] = run;
+      run += x[u];
+    }
+  }
+  for (; k < nchunks; k++) {
     uint32_t x = col[(uint64_t)k * B];
     col[(uint64_t)k * B] = run;
     run += x;
@@ -353,27 +380,19 @@ __global__ void k_zero_u32(uint32_t* __restrict__ out, uint64_t n) {
 
 // sorted[offsets[key] + chunk base + rank] = (key, val), one MSM's entries
 __global__ void k_scatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                          const uint32_t* __restrict__ ranks, uint32_t n, uint32_t Wd, uint32_t B, uint32_t chunk,
+                          const uint32_t* __restrict__ ranks, uint32_t n, uint32_t Wd, uint32_t B, int log2chunk,
                           uint32_t nchunks, uint32_t key_base, int fixed, const uint32_t* __restrict__ hist,
                           const uint32_t* __restrict__ offsets, uint2* __restrict__ sorted) {
   uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (uint64_t)Wd * n) return;
   uint32_t key = keys[e];
   if (key == NOKEY) return;
-  uint32_t val = vals[e];
-  uint32_t kb = key - key_base, idx = val & 0x7FFFFFFFu;
-  uint32_t w, b, i;
-  if (fixed) {
-    w = idx / n;
-    i = idx % n;
-    b = kb;
-  } else {
-    w = kb / B;
-    b = kb % B;
-    i = idx;
-  }
-  uint32_t base = hist[((uint64_t)w * nchunks + i / chunk) * B + b];
-  sorted[offsets[key] + base + ranks[e]] = make_uint2(key, val);
+  uint32_t val = vals[e], rk = ranks[e];
+  uint32_t w = rk >> 24, idx = val & 0x7FFFFFFFu;
+  uint32_t i = fixed ? idx - w * n : idx;
+  uint32_t b = (key - key_base) & (B - 1);
+  uint32_t base = hist[((uint64_t)w * nchunks + (i >> log2chunk)) * B + b];
+  sorted[offsets[key] + base + (rk & 0xFFFFFFu)] = make_uint2(key, val);
 }
 
 __global__ void k_pack16(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint64_t nout) {
@@ -410,13 +429,16 @@ void launch_zero_u32(uint32_t* p, uint64_t n, hipStream_t s) {
   BG_HIP(hipGetLastError());
 }
 
-// chunk of scalars per counting block: at least 16384, at most 32 chunks per MSM, and at
-// least half the bucket count (bounds the chunk histograms for wide windows)
-static uint32_t count_chunk(uint64_t n, uint32_t B) {
-  uint64_t c = std::max<uint64_t>((n + 31) / 32, B / 2);
-  c = (c + 1023) / 1024 * 1024;
-  return (uint32_t)std::max<uint64_t>(16384, c);
+// chunk of scalars per counting block: a power of two of at least 16384, about n/32 (at
+// most 32 chunks) and at least half the bucket count; chunks up to 2^15 take the packed
+// 16-bit histogram
+static int count_chunk_log2(uint64_t n, uint32_t B) {
+  uint64_t want = std::max<uint64_t>((n + 31) / 32, B / 2);
+  int l = 14;
+  while ((1ull << l) < want && l < 24) l++;
+  return l;
 }
+static uint32_t count_chunk(uint64_t n, uint32_t B) { return 1u << count_chunk_log2(n, B); }
 
 uint64_t msm_hist_words(const MsmPlan& p, const uint64_t* ns) {
   uint64_t t = 0;
@@ -434,7 +456,6 @@ void msm_sort(const uint32_t* const* d_scalars, const uint32_t* const* d_points,
               uint32_t* counts, uint32_t* offsets, uint32_t* bsum, uint2* sorted, uint32_t* fix, uint64_t fix_words,
               uint32_t* scal, uint32_t* hist) {
   uint32_t nblk = msm_scan_blocks(p.nb);
-  const uint32_t parts = p.B > HIST_MAX ? p.B / HIST_MAX : 1;
   const int fixed = p.fixed ? 1 : 0;
   {
     ScopedKTime kt(group == 1 ? "msm_g1_digits" : "msm_g2_digits", s);
@@ -447,8 +468,17 @@ void msm_sort(const uint32_t* const* d_scalars, const uint32_t* const* d_points,
         const uint32_t ch = count_chunk(n, p.B), nch = (uint32_t)((n + ch - 1) / ch);
         k_prep_scalars<<<blocks_for(n, 256), 256, 0, s>>>(d_scalars[i], d_points[i], group == 1 ? 16 : 32,
                                                            (uint32_t)n, scal + soff);
-        k_count<<<dim3(nch, p.Wd, parts), COUNT_T, 0, s>>>(scal + soff, (uint32_t)n, p.c, ch, nch, i * p.kspan,
-                                                           fixed, keys + off, vals + off, ranks + off, hist + hoff);
+        const bool p16 = count_chunk_log2(n, p.B) <= PACKED_CHUNK_LOG2;
+        const uint32_t hmax = p16 ? 2 * HIST_WORDS : HIST_WORDS;
+        const uint32_t parts = p.B > hmax ? p.B / hmax : 1;
+        if (p16)
+          k_count<true><<<dim3(nch, p.Wd, parts), COUNT_T, 0, s>>>(scal + soff, (uint32_t)n, p.c, ch, nch,
+                                                                   i * p.kspan, fixed, keys + off, vals + off,
+                                                                   ranks + off, hist + hoff);
+        else
+          k_count<false><<<dim3(nch, p.Wd, parts), COUNT_T, 0, s>>>(scal + soff, (uint32_t)n, p.c, ch, nch,
+                                                                    i * p.kspan, fixed, keys + off, vals + off,
+                                                                    ranks + off, hist + hoff);
         // a bucket's column runs over every (window, chunk) histogram that feeds it
         if (fixed)
           k_column_scan<<<blocks_for(p.B, 256), 256, 0, s>>>(hist + hoff, 1, p.B, (uint32_t)p.Wd * nch,
@@ -474,8 +504,9 @@ void msm_sort(const uint32_t* const* d_scalars, const uint32_t* const* d_points,
       if (n) {
         const uint32_t ch = count_chunk(n, p.B), nch = (uint32_t)((n + ch - 1) / ch);
         k_scatter<<<blocks_for((uint64_t)p.Wd * n, 256), 256, 0, s>>>(keys + off, vals + off, ranks + off,
-                                                                      (uint32_t)n, p.Wd, p.B, ch, nch, i * p.kspan,
-                                                                      fixed, hist + hoff, offsets, sorted);
+                                                                      (uint32_t)n, p.Wd, p.B, count_chunk_log2(n, p.B),
+                                                                      nch, i * p.kspan, fixed, hist + hoff, offsets,
+                                                                      sorted);
         hoff += (uint64_t)p.Wd * nch * p.B;
       }
       off += (uint64_t)p.Wd * n;
