@@ -38,6 +38,11 @@ struct Curve {
     return {a.x, a.y, F::one(), F::one()};
   }
   __device__ __forceinline__ static A aff_neg(const A& a) { return {a.x, F::neg(a.y)}; }
+  // canonical coordinates (values leaving the device path)
+  __device__ __forceinline__ static P canon(const P& p) {
+    return {F::canon(p.X), F::canon(p.Y), F::canon(p.ZZ), F::canon(p.ZZZ)};
+  }
+  __device__ __forceinline__ static A aff_canon(const A& a) { return {F::canon(a.x), F::canon(a.y)}; }
   __device__ __forceinline__ static P neg(const P& p) { return {p.X, F::neg(p.Y), p.ZZ, p.ZZZ}; }
 
   // 2 * affine (a, not infinity): mdbl-2008-s-1

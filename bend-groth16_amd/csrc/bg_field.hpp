@@ -6,6 +6,11 @@
 // a*b + m*p is accumulated in a 96-bit (v_mad_u64_u32 + carry word) accumulator, the
 // reduction digit m_k = lo(acc) * n0 is formed in the same column, and the column is
 // shifted out.  136 32x32->64 multiplies per product (64 a*b, 64 m*p, 8 m digits).
+//
+// Values are kept lazily reduced, in [0, 2p): p < 2^254, so 4p < 2^256 = R and a
+// Montgomery product of inputs below 2p is already below 2p (no final subtraction); add and
+// subtract reduce modulo 2p.  is_zero / eq test congruence mod p; canon() gives the
+// canonical representative, which every value leaving the device path goes through.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -16,11 +21,13 @@ struct Fp {
   uint32_t v[8];
 };
 
-// Field constants.  p < 2^254 for both BN254 fields, so a Montgomery product of inputs < p
-// is < 2p before the final conditional subtraction and never needs a 9th word.
+// Field constants.  p < 2^254 for both BN254 fields: a Montgomery product of inputs < 2p is
+// < 2p and never needs a 9th word.
 struct FrP {
   static constexpr uint32_t p[8] = {0xf0000001u, 0x43e1f593u, 0x79b97091u, 0x2833e848u,
                                     0x8181585du, 0xb85045b6u, 0xe131a029u, 0x30644e72u};
+  static constexpr uint32_t p2[8] = {0xe0000002u, 0x87c3eb27u, 0xf372e122u, 0x5067d090u,
+                                     0x0302b0bau, 0x70a08b6du, 0xc2634053u, 0x60c89ce5u};  // 2p
   static constexpr uint32_t n0 = 0xefffffffu;
   static constexpr uint32_t one[8] = {0x4ffffffbu, 0xac96341cu, 0x9f60cd29u, 0x36fc7695u,
                                       0x7879462eu, 0x666ea36fu, 0x9a07df2fu, 0x0e0a77c1u};
@@ -30,6 +37,8 @@ struct FrP {
 struct FqP {
   static constexpr uint32_t p[8] = {0xd87cfd47u, 0x3c208c16u, 0x6871ca8du, 0x97816a91u,
                                     0x8181585du, 0xb85045b6u, 0xe131a029u, 0x30644e72u};
+  static constexpr uint32_t p2[8] = {0xb0f9fa8eu, 0x7841182du, 0xd0e3951au, 0x2f02d522u,
+                                     0x0302b0bbu, 0x70a08b6du, 0xc2634053u, 0x60c89ce5u};  // 2p
   static constexpr uint32_t n0 = 0xe4866389u;
   static constexpr uint32_t one[8] = {0xc58f0d9du, 0xd35d438du, 0xf5c70b3du, 0x0a78eb28u,
                                       0x7879462cu, 0x666ea36fu, 0x9a07df2fu, 0x0e0a77c1u};
@@ -134,6 +143,18 @@ __device__ __forceinline__ void reduce_once(uint32_t* r, const uint32_t* a) {
   for (int i = 0; i < 8; i++) r[i] = select(b, d[i], a[i]);
 }
 
+// r = a - 2p if a >= 2p else a (a < 4p)
+template <class P>
+__device__ __forceinline__ void reduce_2p(uint32_t* r, const uint32_t* a) {
+  uint32_t d[8];
+  uint64_t b;
+  d[0] = sub_co_k(a[0], P::p2[0], b);
+#pragma unroll
+  for (int i = 1; i < 8; i++) d[i] = subb_co_k(a[i], P::p2[i], b, b);
+#pragma unroll
+  for (int i = 0; i < 8; i++) r[i] = select(b, d[i], a[i]);
+}
+
 // ------------------------------------------------------------------ field ops
 
 template <class P>
@@ -152,17 +173,22 @@ struct Field {
     for (int i = 0; i < 8; i++) r.v[i] = P::one[i];
     return r;
   }
+  // a = 0 mod p (a in [0, 2p): 0 or p)
   __device__ __forceinline__ static bool is_zero(const Fp& a) {
-    uint32_t x = 0;
+    uint32_t x = 0, y = 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) x |= a.v[i];
-    return x == 0;
+    for (int i = 0; i < 8; i++) {
+      x |= a.v[i];
+      y |= a.v[i] ^ P::p[i];
+    }
+    return x == 0 || y == 0;
   }
-  __device__ __forceinline__ static bool eq(const Fp& a, const Fp& b) {
-    uint32_t x = 0;
-#pragma unroll
-    for (int i = 0; i < 8; i++) x |= a.v[i] ^ b.v[i];
-    return x == 0;
+  __device__ __forceinline__ static bool eq(const Fp& a, const Fp& b) { return is_zero(sub(a, b)); }
+  // canonical representative in [0, p)
+  __device__ __forceinline__ static Fp canon(const Fp& a) {
+    Fp r;
+    reduce_once<P>(r.v, a.v);
+    return r;
   }
 
   __device__ __forceinline__ static Fp add(const Fp& a, const Fp& b) {
@@ -172,7 +198,7 @@ struct Field {
 #pragma unroll
     for (int i = 1; i < 8; i++) s[i] = addc_co(a.v[i], b.v[i], c, c);
     Fp r;
-    reduce_once<P>(r.v, s);  // a + b < 2p < 2^255: no carry out of word 7
+    reduce_2p<P>(r.v, s);  // a + b < 4p < 2^256: no carry out of word 7
     return r;
   }
 
@@ -182,10 +208,10 @@ struct Field {
     d[0] = sub_co(a.v[0], b.v[0], br);
 #pragma unroll
     for (int i = 1; i < 8; i++) d[i] = subb_co(a.v[i], b.v[i], br, br);
-    // borrow -> add p back
-    t[0] = add_co_k(d[0], P::p[0], c);
+    // borrow -> add 2p back
+    t[0] = add_co_k(d[0], P::p2[0], c);
 #pragma unroll
-    for (int i = 1; i < 8; i++) t[i] = addc_co_k(d[i], P::p[i], c, c);
+    for (int i = 1; i < 8; i++) t[i] = addc_co_k(d[i], P::p2[i], c, c);
     Fp r;
 #pragma unroll
     for (int i = 0; i < 8; i++) r.v[i] = select(br, d[i], t[i]);
@@ -224,8 +250,9 @@ struct Field {
       acc_shift(acc);
     }
     r[7] = (uint32_t)acc.lo;
-    Fp o;
-    reduce_once<P>(o.v, r);
+    Fp o;  // < 2p for inputs < 2p: no final subtraction
+#pragma unroll
+    for (int i = 0; i < 8; i++) o.v[i] = r[i];
     return o;
   }
 
@@ -299,6 +326,7 @@ struct Fq2 {
   __device__ __forceinline__ static Fp2 one() { return {Fq::one(), Fq::zero()}; }
   __device__ __forceinline__ static bool is_zero(const Fp2& a) { return Fq::is_zero(a.c0) && Fq::is_zero(a.c1); }
   __device__ __forceinline__ static bool eq(const Fp2& a, const Fp2& b) { return Fq::eq(a.c0, b.c0) && Fq::eq(a.c1, b.c1); }
+  __device__ __forceinline__ static Fp2 canon(const Fp2& a) { return {Fq::canon(a.c0), Fq::canon(a.c1)}; }
   __device__ __forceinline__ static Fp2 add(const Fp2& a, const Fp2& b) { return {Fq::add(a.c0, b.c0), Fq::add(a.c1, b.c1)}; }
   __device__ __forceinline__ static Fp2 sub(const Fp2& a, const Fp2& b) { return {Fq::sub(a.c0, b.c0), Fq::sub(a.c1, b.c1)}; }
   __device__ __forceinline__ static Fp2 neg(const Fp2& a) { return {Fq::neg(a.c0), Fq::neg(a.c1)}; }

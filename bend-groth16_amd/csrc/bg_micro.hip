@@ -36,7 +36,7 @@ __global__ void k_kat_field_mul(uint64_t total, uint32_t* out) {
   for (uint64_t i = tid; i < total; i += stride) {
     Fp a = kat_rnd((uint32_t)(2 * i));
     Fp b = kat_rnd((uint32_t)(2 * i + 1));
-    Fp c = Fr::mul(a, b);
+    Fp c = Fr::canon(Fr::mul(a, b));
     x ^= c.v[0] & 0xFFFFu;
   }
   // wave-level xor reduction, then one atomic per wave
@@ -51,7 +51,7 @@ __global__ void k_kat_g1_chain(uint64_t k, uint32_t* out) {
   Aff<Fq> g = {Fq::one(), Fq::add(Fq::one(), Fq::one())};
   J::J acc = {g.x, g.y, Fq::one()};
   for (uint64_t i = 0; i < k; i++) acc = J::madd(acc, g);
-  out[0] = acc.x.v[0] & 0xFFFFu;
+  out[0] = Fq::canon(acc.x).v[0] & 0xFFFFu;
 }
 
 // ---------------------------------------------------------------- element-wise ops
@@ -89,10 +89,10 @@ __global__ void k_field_op(int field, int op, const uint32_t* a, const uint32_t*
   if (i >= n) return;
   if (field == 2) {
     Fp2 x = load_fp2(a + 16 * i), y = load_fp2(b + 16 * i);
-    store_fp2(out + 16 * i, fp2_apply(op, x, y));
+    store_fp2(out + 16 * i, Fq2::canon(fp2_apply(op, x, y)));
   } else {
     Fp x = load_fp(a + 8 * i), y = load_fp(b + 8 * i);
-    Fp r = field == 0 ? fp_apply<FrP>(op, x, y) : fp_apply<FqP>(op, x, y);
+    Fp r = field == 0 ? Fr::canon(fp_apply<FrP>(op, x, y)) : Fq::canon(fp_apply<FqP>(op, x, y));
     store_fp(out + 8 * i, r);
   }
 }
@@ -112,7 +112,7 @@ __global__ void k_curve_op(int op, const uint32_t* p, const uint32_t* q, uint32_
     case 2: r = C::from_affine(C::aff_neg(a)); break;
     default: r = C::add(C::from_affine(a), C::madd(C::infinity(), b)); break;
   }
-  C::store_aff(out + aw * i, C::to_affine(r));
+  C::store_aff(out + aw * i, C::aff_canon(C::to_affine(r)));
 }
 
 // ---------------------------------------------------------------- throughput probes

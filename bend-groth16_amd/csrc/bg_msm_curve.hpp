@@ -373,8 +373,9 @@ __global__ __launch_bounds__(256) void k_reduce2(const uint32_t* __restrict__ lv
       P V = shA[0];
       for (int i = 0; i < log2m; i++) V = C::dbl(V);
       P out = CAREFUL ? C::add(shU[0], V) : C::add_lite(shU[0], V, bad);
-      C::store(outPQ + (uint64_t)C::xyzz_words * (2 * blk), out);
-      C::store(outPQ + (uint64_t)C::xyzz_words * (2 * blk + 1), Z);
+      // canonical coordinates: the host finishes the sums
+      C::store(outPQ + (uint64_t)C::xyzz_words * (2 * blk), C::canon(out));
+      C::store(outPQ + (uint64_t)C::xyzz_words * (2 * blk + 1), C::canon(Z));
     }
     if (!CAREFUL && bad) atomicOr(&blk_bad, 1u);
     __syncthreads();

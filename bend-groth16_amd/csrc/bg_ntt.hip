@@ -40,7 +40,7 @@ __global__ void k_gather(const uint32_t* __restrict__ in, uint32_t* __restrict__
     x = Fr::mul(x, load_fp(scale + 8ull * k));
   else if (use_c)
     x = Fr::mul(x, c);
-  store_fp(out + 8ull * k, x);
+  store_fp(out + 8ull * k, Fr::canon(x));  // transform outputs leave the device path canonical
 }
 
 // One pass: radix-2 stages k = k0 .. k0+t-1 (half-size m = 2^(log2-1-k)).  The stages

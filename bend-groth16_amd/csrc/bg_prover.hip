@@ -114,7 +114,7 @@ __global__ void k_evals(const uint32_t* __restrict__ offs, const uint32_t* __res
 
 __global__ void k_from_mont(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint64_t n) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) store_fp(out + 8 * i, Fr::from_mont(load_fp(in + 8 * i)));
+  if (i < n) store_fp(out + 8 * i, Fr::canon(Fr::from_mont(load_fp(in + 8 * i))));
 }
 
 static unsigned nb(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
