@@ -84,6 +84,8 @@ def lib() -> ctypes.CDLL:
         "bg_prove_batch": [I, P, P, P, P, I, I],
         "bg_fixed_base_mul": [I, P, U64, P, I],
         "bg_gen_bases": [I, U64, U64, P, P, I],
+        "bg_verify": [P, U64, P, P, ctypes.c_uint32, I],
+        "bg_pairing_check": [P, P, ctypes.c_uint32, I],
     }
     for name, args in sigs.items():
         fn = getattr(L, name, None)
@@ -272,6 +274,27 @@ def gen_bases(group: int, seed: int, n: int, with_k: bool = False, threads: int 
     k = np.zeros((n, 8), dtype=np.uint32) if with_k else None
     _check(lib().bg_gen_bases(group, seed, n, _p(out), _p(k), threads))
     return (out, k) if with_k else out
+
+
+def verify(vk: np.ndarray, proof: np.ndarray, inputs: np.ndarray, fmt: str = "words16") -> bool:
+    """Groth16 check e(A, B) = e(alpha, beta) e(IC(x), gamma) e(C, delta) (ref/src/verify.rs:8-27).
+
+    vk: the words of a vk.bin; proof: A, B, C (the prover's output encoding); inputs: the
+    public inputs as Montgomery Fr.  False for a proof that fails the equation; BgError for
+    a point off its curve or outside G2 (arkworks refuses those too)."""
+    v, pr, x = _u32(vk).reshape(-1), _u32(proof).reshape(-1), _u32(inputs).reshape(-1)
+    ew = 16 if fmt == "words16" else 8
+    return _check(lib().bg_verify(_p(v), v.size, _p(pr), _p(x), x.size // ew, _fmt_flag(fmt))) == 1
+
+
+def pairing_check(g1s: np.ndarray, g2s: np.ndarray, fmt: str = "packed") -> bool:
+    """prod_i e(P_i, Q_i) == 1 for affine G1 points P_i and G2 points Q_i (Montgomery)."""
+    a, b = _u32(g1s).reshape(-1), _u32(g2s).reshape(-1)
+    ew = 16 if fmt == "words16" else 8
+    n = a.size // (2 * ew)
+    if b.size != 4 * ew * n:
+        raise ValueError("g1s and g2s hold different point counts")
+    return _check(lib().bg_pairing_check(_p(a), _p(b), n, _fmt_flag(fmt))) == 1
 
 
 def msm_plan_info(n: int, group: int = 1, window_bits: int = 0) -> dict:

@@ -312,6 +312,31 @@ extern "C" int bg_setup_square_chain(const char* dir, int log2, uint64_t seed, i
     g2_file("pk_b2.bin", v);
     g1_file("pk_h.bin", hs);
     g1_file("pk_l.bin", ls);
+    {
+      // verifying key (ref/src/gen.rs:159-164 keeps arkworks' vk; this library's verifier reads
+      // vk.bin): alpha G1, beta G2, gamma G2, delta G2, count, gamma_abc[count] with
+      // gamma_abc_i = (beta u_i + alpha v_i + w_i) / gamma G1 for the instance variables
+      Fr gam_inv = gamma.inv();
+      std::vector<Fr> k1 = {alpha};
+      for (uint64_t i = 0; i < ni; i++) k1.push_back((beta * u[i] + alpha * v[i] + w[i]) * gam_inv);
+      std::vector<Fq> ax, ay;
+      std::vector<Fq2> bx, by;
+      fixed_base(g1, k1, ax, ay, 1);
+      fixed_base(g2, std::vector<Fr>{beta, gamma, delta}, bx, by, 1);
+      std::vector<uint32_t> out;
+      put_fq(out, ax[0]);
+      put_fq(out, ay[0]);
+      for (int i = 0; i < 3; i++) {
+        put_fq2(out, bx[i]);
+        put_fq2(out, by[i]);
+      }
+      out.push_back((uint32_t)ni);
+      for (uint64_t i = 0; i < ni; i++) {
+        put_fq(out, ax[1 + i]);
+        put_fq(out, ay[1 + i]);
+      }
+      W.file("vk.bin", out);
+    }
     std::vector<uint32_t> hdr = {(uint32_t)N,      (uint32_t)ni,     (uint32_t)nw,      (uint32_t)lg,
                                  (uint32_t)nz,     (uint32_t)nz,     (uint32_t)(d - 1), (uint32_t)nw};
     W.file("header.bin", hdr);

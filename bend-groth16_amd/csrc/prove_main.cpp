@@ -2,6 +2,8 @@
 // :129-162).  The key directory comes from argv[1] or $BENDG_DATA (FORMAT.md files);
 // stdout is "A x y", "B x0 x1 y0 y1", "C x y" (canonical big-endian hex) followed by
 // "T <phase> <ms>" lines for load evals qap msm_a msm_b1 msm_b2 msm_h msm_l total.
+// With --verify the proof is also checked against <key dir>/vk.bin with the public input
+// z_1 (bg_verify): "V ok" on stdout, or exit status 1.
 #include "../../include/bg16.h"
 #include <chrono>
 #include <cstdio>
@@ -29,9 +31,16 @@ static void die(const char* what, int rc) {
 }
 
 int main(int argc, char** argv) {
-  const char* dir = argc > 1 ? argv[1] : getenv("BENDG_DATA");
+  const char* dir = getenv("BENDG_DATA");
+  bool verify = false;
+  for (int i = 1; i < argc; i++) {
+    if (std::string(argv[i]) == "--verify")
+      verify = true;
+    else
+      dir = argv[i];
+  }
   if (!dir) {
-    fprintf(stderr, "usage: bg16_prove <key dir>   (or set BENDG_DATA)\n");
+    fprintf(stderr, "usage: bg16_prove [--verify] <key dir>   (or set BENDG_DATA)\n");
     return 2;
   }
   auto t0 = std::chrono::steady_clock::now();
@@ -65,5 +74,15 @@ int main(int argc, char** argv) {
          "T msm_l %.3f\nT prove %.3f\nT total %.3f\n",
          load, t[0] + t[1], t[2], t[3], t[4], t[5], t[6], t[7], t[8], total);
   bg_pk_free(pk);
+  if (verify) {
+    auto vk = read_all(d + "/vk.bin");
+    rc = bg_verify(vk.data(), vk.size(), proof16.data(), z.data() + 16, hdr[1] - 1, BG_FMT_WORDS16);
+    if (rc < 0) die("bg_verify", rc);
+    if (rc != 1) {
+      fprintf(stderr, "bg16_prove: the proof does not verify against vk.bin\n");
+      return 1;
+    }
+    printf("V ok\n");
+  }
   return 0;
 }

@@ -170,6 +170,20 @@ int bg_fixed_base_mul(int group, const uint32_t* scalars, uint64_t n, uint32_t* 
  * primitive MSM inputs); k_out (optional) receives the k_i canonical packed. */
 int bg_gen_bases(int group, uint64_t seed, uint64_t n, uint32_t* out_affine, uint32_t* k_out, int threads);
 
+/* ------------------------------------------------------------------ verification (host)
+ * Replaces ref/src/verify.rs:8-27 (arkworks prepare_verifying_key + verify_proof).
+ * vk: the words of vk.bin (bg_setup_square_chain): alpha G1, beta G2, gamma G2, delta G2,
+ * a u32 count, gamma_abc G1[count]; proof as bg_prove writes it; inputs: n_inputs
+ * Montgomery Fr (count must be n_inputs + 1).  flags: BG_FMT_*.  Returns 1 when
+ * e(A, B) = e(alpha, beta) e(IC(x), gamma) e(C, delta), 0 when not, BG_ERR_ARG when a
+ * point is off its curve, B / a vk G2 point is outside the order-r subgroup, or a
+ * coordinate is not below q (the reference's json.rs check_point errors on those). */
+int bg_verify(const uint32_t* vk, uint64_t vk_words, const uint32_t* proof, const uint32_t* inputs,
+              uint32_t n_inputs, int flags);
+/* prod_{i<n} e(P_i, Q_i) == 1 (optimal ate on BN254): 1 yes, 0 no, BG_ERR_ARG for a
+ * bad point.  g1s: n affine G1, g2s: n affine G2 (Montgomery, infinity = all zero). */
+int bg_pairing_check(const uint32_t* g1s, const uint32_t* g2s, uint32_t n, int flags);
+
 /* ------------------------------------------------------------------ diagnostics
  * Known-answer and element-wise kernels used by the parity tests (host buffers, packed). */
 

@@ -44,6 +44,10 @@ def test_prove_vs_oracle_and_closed_form(bg, keys, lg):
     assert np.array_equal(A, oA) and np.array_equal(B, oB) and np.array_equal(C, oC)
     eA, eB, eC = points_of(closed_form(inst))
     assert np.array_equal(A, eA) and np.array_equal(B, eB) and np.array_equal(C, eC)
+    # the device proof passes the pairing equation (bg_verify, host)
+    vk = np.fromfile(d / "vk.bin", dtype=np.uint32)
+    assert bendg.verify(vk, proof16, z[16:32])
+    assert not bendg.verify(vk, proof16, z[32:48])
     pk.close()
 
 
@@ -98,6 +102,7 @@ def test_prove_large_closed_form(bg, tmp_path, lg):
     print("closed form", flush=True)
     eA, eB, eC = points_of(closed_form(load(tmp_path)))
     assert np.array_equal(A, eA) and np.array_equal(B, eB) and np.array_equal(C, eC)
+    assert bendg.verify(np.fromfile(tmp_path / "vk.bin", dtype=np.uint32), proof16, z[16:32])
     pk.close()
 
 
@@ -120,3 +125,23 @@ def test_prove_windowed_and_fixed_base_agree(bg, keys, monkeypatch):
     assert np.array_equal(A, oA) and np.array_equal(B, oB) and np.array_equal(C, oC)
     pk_fixed.close()
     pk_plain.close()
+
+
+def test_cli_prove_and_verify(bg, keys):
+    """bg16_prove (the bend2/prove.bend command-line shape) prints the library's proof and,
+    with --verify, checks it against vk.bin."""
+    import subprocess
+    from pathlib import Path
+    cli = Path(bendg.__file__).resolve().parent.parent / "bg16_prove"
+    d = keys[10]
+    out = subprocess.run([str(cli), "--verify", str(d)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.strip().split("\n")
+    pk = bendg.ProvingKey(d)
+    z, rs = _files(d)
+    proof16, _ = pk.prove(z, rs, fmt="words16")
+    pk.close()
+    want = bendg.format_proof(bendg.pack16(proof16.reshape(-1, 16)).reshape(-1)).strip().split("\n")
+    assert lines[:3] == want
+    assert lines[-1] == "V ok"
+    assert any(l.startswith("T total ") for l in lines)
