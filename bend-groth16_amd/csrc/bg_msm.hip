@@ -12,6 +12,13 @@ namespace bg {
 
 static constexpr uint32_t NOKEY = 0xFFFFFFFFu;
 
+// counting sort (k_count): LDS histogram words (128 KB of 16-bit counter pairs), buckets
+// one block counts, threads per block, largest chunk of scalars per block
+static constexpr uint32_t HIST_WORDS = 32768;
+static constexpr uint32_t HIST_BUCKETS = 2 * HIST_WORDS;
+static constexpr int COUNT_T = 1024;
+static constexpr int MAX_CHUNK_LOG2 = 15;
+
 // ------------------------------------------------------------------ planning
 
 // Tuning overrides (BG16_MSM_C_G1 / BG16_MSM_C_G2 / BG16_MSM_L / BG16_MSM_M / BG16_MSM_K);
@@ -59,6 +66,24 @@ static void finish_plan(MsmPlan& p, int group) {
   p.acc_threads = (uint32_t)((p.entries + p.L - 1) / p.L);
 }
 
+// Window-width cost in tenths of a mixed add: accumulation (10 per entry), digit counting
+// (about 1.1 per entry per bucket part: windows wider than 17 bits are counted in several
+// passes), chunk histograms (about 0.65 per word: written, scanned, offset, read by the
+// scatter) and the bucket reduction (28 per bucket).  Weights from measured kernel times.
+static double window_cost(const uint64_t* ns, uint32_t M, int c, uint32_t Wd, uint32_t Wb) {
+  const uint64_t B = 1ull << (c - 1);
+  const double parts = B > HIST_BUCKETS ? (double)(B / HIST_BUCKETS) : 1.0;
+  double cost = (double)M * Wb * (double)B * 28.0;
+  for (uint32_t i = 0; i < M; i++) {
+    if (!ns[i]) continue;
+    const uint64_t want = std::max<uint64_t>((ns[i] + 31) / 32, B / 2);
+    const uint64_t ch = want > (1ull << 14) ? (1ull << MAX_CHUNK_LOG2) : (1ull << 14);
+    const double hist_words = (double)Wd * (double)((ns[i] + ch - 1) / ch) * (double)B;
+    cost += (double)ns[i] * Wd * (10.0 + 1.1 * parts) + 0.65 * hist_words;
+  }
+  return cost;
+}
+
 MsmPlan msm_plan_batch(const uint64_t* ns, uint32_t M, int group, int window_bits) {
   MsmPlan p;
   if (window_bits <= 0) window_bits = tune_env(group == 1 ? "BG16_MSM_C_G1" : "BG16_MSM_C_G2");
@@ -68,11 +93,11 @@ MsmPlan msm_plan_batch(const uint64_t* ns, uint32_t M, int group, int window_bit
   p.n = n;
   int c = window_bits;
   if (c <= 0) {
-    // minimise n*W*madd + M*W*2^(c-1)*2*add in field products (G1 10/14; G2 has the same ratio)
+    // every window has its own buckets
     double best = 1e300;
     for (int cc = 4; cc <= 20; cc++) {
-      int W = (255 + cc - 1) / cc;
-      double cost = (double)n * W * 10.0 + (double)M * W * (double)(1u << (cc - 1)) * 28.0;
+      const uint32_t W = (255 + cc - 1) / cc;
+      const double cost = window_cost(ns, M, cc, W, W);
       if (cost < best) {
         best = cost;
         c = cc;
@@ -88,14 +113,12 @@ MsmPlan msm_plan_batch(const uint64_t* ns, uint32_t M, int group, int window_bit
 int fixed_window_bits(const uint64_t* ns, uint32_t M, int group) {
   int env = tune_env(group == 1 ? "BG16_FIXED_C_G1" : "BG16_FIXED_C_G2");
   if (env > 0) return env;
-  uint64_t n = 0;
-  for (uint32_t i = 0; i < M; i++) n += ns[i];
-  // every window shares one set of buckets: minimise n*Wd*madd + M*2^(c-1)*2*add
+  // every window shares one set of buckets
   double best = 1e300;
   int c = 16;
   for (int cc = 8; cc <= 20; cc++) {
-    int Wd = (255 + cc - 1) / cc;
-    double cost = (double)n * Wd * 10.0 + (double)M * (double)(1u << (cc - 1)) * 28.0;
+    const uint32_t Wd = (255 + cc - 1) / cc;
+    const double cost = window_cost(ns, M, cc, Wd, 1);
     if (cost < best) {
       best = cost;
       c = cc;
@@ -209,24 +232,20 @@ __device__ __forceinline__ uint32_t window_carry(const uint32_t* s, const uint32
 // histogram (see k_count), writes (key, index | sign, rank in block) for
 // each entry it owns and its histogram to hist[(window * nchunks + chunk) * B + bucket];
 // k_column_scan turns the histograms into per-chunk bucket bases and bucket totals.
-// LDS histogram of 64 KB: with chunks of at most 2^15 scalars the counters are 16-bit,
-// two to a word (32768 buckets per block; no counter reaches 2^16); larger chunks use
-// 32-bit counters (16384 buckets per block).  Wider windows split the bucket range over
-// blockIdx.z.  The rank word also carries the window in bits 24..31, so the scatter finds
-// the entry's chunk histogram without dividing.
-static constexpr uint32_t HIST_WORDS = 16384;
-static constexpr int COUNT_T = 1024;
-static constexpr int PACKED_CHUNK_LOG2 = 15;
+// The histogram takes 128 KB of gfx950's 160 KB LDS: 65536 16-bit counters, two to a
+// word.  Chunks hold at most 2^15 scalars, so no counter reaches 2^16 even when every
+// scalar of a chunk lands in one bucket.  Windows wider than 17 bits split the bucket range
+// over blockIdx.z (each part reads the chunk again; the window planners charge for it).
+// The rank word also carries the window in bits 24..31, so the scatter finds the entry's
+// chunk histogram without dividing.
 
-template <bool P16>
 __global__ __launch_bounds__(COUNT_T) void k_count(const uint32_t* __restrict__ sc, uint32_t n, int c,
                                                    uint32_t chunk, uint32_t nchunks, uint32_t key_base, int fixed,
                                                    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
                                                    uint32_t* __restrict__ ranks, uint32_t* __restrict__ hist_out) {
   __shared__ uint32_t hist[HIST_WORDS];
-  constexpr uint32_t HMAX = P16 ? 2 * HIST_WORDS : HIST_WORDS;
   const uint32_t B = 1u << (c - 1);
-  const uint32_t Bp = B < HMAX ? B : HMAX;
+  const uint32_t Bp = B < HIST_BUCKETS ? B : HIST_BUCKETS;
   const uint32_t w = blockIdx.y;
   const uint32_t lo_b = blockIdx.z * Bp;
   const int nbits = (int)w * c;
@@ -235,7 +254,7 @@ __global__ __launch_bounds__(COUNT_T) void k_count(const uint32_t* __restrict__ 
     int pos = j * c + c - 1;
     if (pos < 256) pat[pos >> 5] |= 1u << (pos & 31);
   }
-  const uint32_t hw = P16 ? (Bp + 1) / 2 : Bp;
+  const uint32_t hw = (Bp + 1) / 2;
   for (uint32_t b = threadIdx.x; b < hw; b += COUNT_T) hist[b] = 0;
   __syncthreads();
   const uint64_t i0 = (uint64_t)blockIdx.x * chunk;
@@ -260,12 +279,8 @@ __global__ __launch_bounds__(COUNT_T) void k_count(const uint32_t* __restrict__ 
         const uint32_t lb = bk - lo_b;
         key = key_base + (fixed ? 0u : w * B) + bk;
         val = (fixed ? w * n + (uint32_t)i : (uint32_t)i) | (neg ? 0x80000000u : 0u);
-        if (P16) {
-          const uint32_t sh = 16 * (lb & 1u);
-          r = (atomicAdd(&hist[lb >> 1], 1u << sh) >> sh) & 0xFFFFu;
-        } else {
-          r = atomicAdd(&hist[lb], 1u);
-        }
+        const uint32_t sh = 16 * (lb & 1u);
+        r = (atomicAdd(&hist[lb >> 1], 1u << sh) >> sh) & 0xFFFFu;
       }
     }
     if (owner) {
@@ -277,8 +292,7 @@ __global__ __launch_bounds__(COUNT_T) void k_count(const uint32_t* __restrict__ 
   }
   __syncthreads();
   uint32_t* ho = hist_out + ((uint64_t)w * nchunks + blockIdx.x) * B + lo_b;
-  for (uint32_t b = threadIdx.x; b < Bp; b += COUNT_T)
-    ho[b] = P16 ? (hist[b >> 1] >> (16 * (b & 1u))) & 0xFFFFu : hist[b];
+  for (uint32_t b = threadIdx.x; b < Bp; b += COUNT_T) ho[b] = (hist[b >> 1] >> (16 * (b & 1u))) & 0xFFFFu;
 }
 
 // For every (window, bucket) of one MSM: exclusive prefix of the chunk histograms in
@@ -442,14 +456,11 @@ void launch_zero_u32(uint32_t* p, uint64_t n, hipStream_t s) {
   BG_HIP(hipGetLastError());
 }
 
-// chunk of scalars per counting block: a power of two of at least 16384, about n/32 (at
-// most 32 chunks) and at least half the bucket count; chunks up to 2^15 take the packed
-// 16-bit histogram
+// chunk of scalars per counting block: 2^14, or 2^15 when n/32 or half the bucket count
+// is larger (the 16-bit LDS counters bound it at 2^15)
 static int count_chunk_log2(uint64_t n, uint32_t B) {
   uint64_t want = std::max<uint64_t>((n + 31) / 32, B / 2);
-  int l = 14;
-  while ((1ull << l) < want && l < 24) l++;
-  return l;
+  return want > (1ull << 14) ? MAX_CHUNK_LOG2 : 14;
 }
 static uint32_t count_chunk(uint64_t n, uint32_t B) { return 1u << count_chunk_log2(n, B); }
 
@@ -481,17 +492,9 @@ void msm_sort(const uint32_t* const* d_scalars, const uint32_t* const* d_points,
         const uint32_t ch = count_chunk(n, p.B), nch = (uint32_t)((n + ch - 1) / ch);
         k_prep_scalars<<<blocks_for(n, 256), 256, 0, s>>>(d_scalars[i], d_points[i], group == 1 ? 16 : 32,
                                                            (uint32_t)n, scal + soff);
-        const bool p16 = count_chunk_log2(n, p.B) <= PACKED_CHUNK_LOG2;
-        const uint32_t hmax = p16 ? 2 * HIST_WORDS : HIST_WORDS;
-        const uint32_t parts = p.B > hmax ? p.B / hmax : 1;
-        if (p16)
-          k_count<true><<<dim3(nch, p.Wd, parts), COUNT_T, 0, s>>>(scal + soff, (uint32_t)n, p.c, ch, nch,
-                                                                   i * p.kspan, fixed, keys + off, vals + off,
-                                                                   ranks + off, hist + hoff);
-        else
-          k_count<false><<<dim3(nch, p.Wd, parts), COUNT_T, 0, s>>>(scal + soff, (uint32_t)n, p.c, ch, nch,
-                                                                    i * p.kspan, fixed, keys + off, vals + off,
-                                                                    ranks + off, hist + hoff);
+        const uint32_t parts = p.B > HIST_BUCKETS ? p.B / HIST_BUCKETS : 1;
+        k_count<<<dim3(nch, p.Wd, parts), COUNT_T, 0, s>>>(scal + soff, (uint32_t)n, p.c, ch, nch, i * p.kspan,
+                                                           fixed, keys + off, vals + off, ranks + off, hist + hoff);
         // a bucket's column runs over every (window, chunk) histogram that feeds it
         if (fixed)
           k_column_scan<<<blocks_for(p.B, 256), 256, 0, s>>>(hist + hoff, 1, p.B, (uint32_t)p.Wd * nch,
