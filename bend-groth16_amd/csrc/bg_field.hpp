@@ -2,10 +2,10 @@
 // Montgomery form with R = 2^256 (the same integers as the reference's 16 x 16-bit limbs,
 // bend2/FORMAT.md:7-8, so repacking is a pure bit pack).
 //
-// The Montgomery product uses finely integrated product scanning: every column of
-// a*b + m*p is accumulated in a 96-bit (v_mad_u64_u32 + carry word) accumulator, the
-// reduction digit m_k = lo(acc) * n0 is formed in the same column, and the column is
-// shifted out.  136 32x32->64 multiplies per product (64 a*b, 64 m*p, 8 m digits).
+// The Montgomery product (bg_mul29.hpp) splits its operands into 29-bit digits and
+// accumulates every column of a*b + m*p in one 64-bit v_mad_u64_u32 accumulator, the
+// reduction digit formed in the same column: 171 32x32->64 multiplies (81 a*b, 81 m*p,
+// 9 m digits) and no carry folds.
 //
 // Values are kept lazily reduced, in [0, 2p): p < 2^254, so 4p < 2^256 = R and a
 // Montgomery product of inputs below 2p is already below 2p (no final subtraction); add and
@@ -14,6 +14,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "bg_mul29.hpp"
 
 namespace bg {
 
@@ -33,6 +34,10 @@ struct FrP {
                                       0x7879462eu, 0x666ea36fu, 0x9a07df2fu, 0x0e0a77c1u};
   static constexpr uint32_t r2[8] = {0xae216da7u, 0x1bb8e645u, 0xe35c59e3u, 0x53fe3ab1u,
                                      0x53bb8085u, 0x8c49833du, 0x7f4e44a5u, 0x0216d0b1u};
+  // p in 29-bit digits and -p^-1 mod 2^29 (the product's digit form)
+  static constexpr uint32_t p29[9] = {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u,
+                                      0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu};
+  static constexpr uint32_t n0_29 = 0x0fffffffu;
 };
 struct FqP {
   static constexpr uint32_t p[8] = {0xd87cfd47u, 0x3c208c16u, 0x6871ca8du, 0x97816a91u,
@@ -44,39 +49,12 @@ struct FqP {
                                       0x7879462cu, 0x666ea36fu, 0x9a07df2fu, 0x0e0a77c1u};
   static constexpr uint32_t r2[8] = {0x538afa89u, 0xf32cfc5bu, 0xd44501fbu, 0xb5e71911u,
                                      0x0a417ff6u, 0x47ab1effu, 0xcab8351fu, 0x06d89f71u};
+  static constexpr uint32_t p29[9] = {0x187cfd47u, 0x010460b6u, 0x1c72a34fu, 0x02d522d0u, 0x1585d978u,
+                                      0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu};
+  static constexpr uint32_t n0_29 = 0x04866389u;
 };
 
 // ------------------------------------------------------------------ carry helpers
-
-// 96-bit column accumulator: lo (64 bits) + hi (32 bits).
-struct Acc {
-  uint64_t lo;
-  uint32_t hi;
-};
-
-// acc += a * b: one v_mad_u64_u32 (64-bit accumulate, carry out to an SGPR pair) and one
-// v_addc_co_u32 folding that carry into the high word.  ALU-only inline assembly: no
-// memory operand, so no load can be left in flight.
-__device__ __forceinline__ void acc_mad(Acc& acc, uint32_t a, uint32_t b) {
-  uint64_t cc;
-  asm("v_mad_u64_u32 %0, %1, %3, %4, %0\n\t"
-      "v_addc_co_u32_e64 %2, %1, %2, 0, %1"
-      : "+v"(acc.lo), "=&s"(cc), "+v"(acc.hi)
-      : "v"(a), "v"(b));
-}
-// same with the second factor a wave-uniform constant held in an SGPR
-__device__ __forceinline__ void acc_mad_s(Acc& acc, uint32_t a, uint32_t k) {
-  uint64_t cc;
-  asm("v_mad_u64_u32 %0, %1, %3, %4, %0\n\t"
-      "v_addc_co_u32_e64 %2, %1, %2, 0, %1"
-      : "+v"(acc.lo), "=&s"(cc), "+v"(acc.hi)
-      : "v"(a), "s"(k));
-}
-
-__device__ __forceinline__ void acc_shift(Acc& acc) {
-  acc.lo = (acc.lo >> 32) | ((uint64_t)acc.hi << 32);
-  acc.hi = 0;
-}
 
 // 32-bit add / subtract steps with the carry or borrow in an SGPR pair (VOP3 forms), so a
 // 256-bit chain is one instruction per limb; the compiler otherwise builds each limb from
@@ -222,37 +200,13 @@ struct Field {
 
   __device__ __forceinline__ static Fp dbl(const Fp& a) { return add(a, a); }
 
-  // Montgomery product a * b * 2^-256 mod p, finely integrated product scanning.
+  // Montgomery product a * b * 2^-256 mod p with 29-bit digits (bg_mul29.hpp): no carry
+  // passes through scalar registers, so the product issues ~250 VALU instructions and no
+  // hazard wait states (a 32-bit-digit product scanning with the carries folded through
+  // SGPR pairs: ~295 VALU instructions plus ~115 s_nop).
   __device__ __forceinline__ static Fp mul(const Fp& a, const Fp& b) {
-    uint32_t m[8];
-    uint32_t r[8];
-    Acc acc = {0, 0};
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-#pragma unroll
-      for (int i = 0; i < k; i++) {
-        acc_mad(acc, a.v[i], b.v[k - i]);
-        acc_mad_s(acc, m[i], P::p[k - i]);
-      }
-      acc_mad(acc, a.v[k], b.v[0]);
-      m[k] = (uint32_t)acc.lo * P::n0;
-      acc_mad_s(acc, m[k], P::p[0]);
-      acc_shift(acc);
-    }
-#pragma unroll
-    for (int k = 8; k < 15; k++) {
-#pragma unroll
-      for (int i = k - 7; i < 8; i++) {
-        acc_mad(acc, a.v[i], b.v[k - i]);
-        acc_mad_s(acc, m[i], P::p[k - i]);
-      }
-      r[k - 8] = (uint32_t)acc.lo;
-      acc_shift(acc);
-    }
-    r[7] = (uint32_t)acc.lo;
-    Fp o;  // < 2p for inputs < 2p: no final subtraction
-#pragma unroll
-    for (int i = 0; i < 8; i++) o.v[i] = r[i];
+    Fp o;
+    m29::mul_words(a.v, b.v, o.v, P::p29, P::n0_29);
     return o;
   }
 

@@ -1,0 +1,105 @@
+// Montgomery product a * b * 2^-256 mod p for 8 x 32-bit word operands, computed with
+// 29-bit digits (reduced-radix product scanning).
+//
+// Each operand is split into 9 digits of 29 bits (a is taken as a * 2^5, so that a
+// radix-2^29 Montgomery reduction, which divides by 2^261, yields a * b * 2^-256).  A
+// digit product is < 2^58, so a column of up to 18 of them plus the carried-in column
+// fits one 64-bit accumulator: every partial product is one v_mad_u64_u32 with no carry
+// fold, and no carry passes through scalar registers.  The reduction digit of column k is
+// m_k = (acc * -p^-1) mod 2^29, formed in the same column.  The result is packed back to
+// 8 words.
+//
+// Range: for a, b < 2p (p < 2^254): a * 2^5 < 2^261 (9 digits), and the result
+// t < (32a * b) / 2^261 + p < 128 p^2 / 2^261 + p < 2p, so values stay in [0, 2p) with no
+// final subtraction, as the 32-bit-digit product it replaces.
+//
+// Plain C++ on purpose: the same code runs on the host in the unit test of the digit
+// arithmetic.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__) || defined(__HIP__)
+#define BG_HD __host__ __device__ __forceinline__
+#else
+#define BG_HD inline
+#endif
+
+namespace bg {
+namespace m29 {
+
+static constexpr uint32_t MASK = (1u << 29) - 1;
+
+struct Digits {
+  uint32_t d[9];
+};
+
+// digits of w * 2^S (S = 0 or 5) for a 256-bit w (w * 2^S < 2^261)
+template <int S>
+BG_HD Digits split(const uint32_t* w) {
+  Digits r;
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    const int b = 29 * i - S;  // first bit of digit i in w
+    uint32_t v;
+    if (b < 0) {
+      v = w[0] << (-b);
+    } else {
+      const int wi = b >> 5, sh = b & 31;
+      v = w[wi] >> sh;
+      if (sh > 3 && wi + 1 < 8) v |= w[wi + 1] << (32 - sh);
+    }
+    r.d[i] = v & MASK;
+  }
+  return r;
+}
+
+// 8 words of a 9-digit value below 2^256 (digits below 2^29)
+BG_HD void pack(const Digits& a, uint32_t* w) {
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const int b = 32 * j, li = b / 29, sh = b % 29;
+    uint32_t v = a.d[li] >> sh;
+    if (li + 1 < 9) v |= a.d[li + 1] << (29 - sh);
+    if (sh > 26 && li + 2 < 9) v |= a.d[li + 2] << (58 - sh);
+    w[j] = v;
+  }
+}
+
+// x * y * 2^-261 mod p (not reduced below p); p29: p in 29-bit digits, n0 = -p^-1 mod 2^29
+BG_HD Digits mont(const Digits& x, const Digits& y, const uint32_t* p29, uint32_t n0) {
+  uint32_t m[9];
+  Digits r;
+  uint64_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+#pragma unroll
+    for (int i = 0; i < k; i++) {
+      acc += (uint64_t)x.d[i] * y.d[k - i];
+      acc += (uint64_t)m[i] * p29[k - i];
+    }
+    acc += (uint64_t)x.d[k] * y.d[0];
+    m[k] = ((uint32_t)acc * n0) & MASK;
+    acc += (uint64_t)m[k] * p29[0];
+    acc >>= 29;
+  }
+#pragma unroll
+  for (int k = 9; k < 17; k++) {
+#pragma unroll
+    for (int i = k - 8; i < 9; i++) {
+      acc += (uint64_t)x.d[i] * y.d[k - i];
+      acc += (uint64_t)m[i] * p29[k - i];
+    }
+    r.d[k - 9] = (uint32_t)acc & MASK;
+    acc >>= 29;
+  }
+  r.d[8] = (uint32_t)acc;
+  return r;
+}
+
+// a * b * 2^-256 mod p on 8-word operands
+BG_HD void mul_words(const uint32_t* a, const uint32_t* b, uint32_t* out, const uint32_t* p29, uint32_t n0) {
+  pack(mont(split<5>(a), split<0>(b), p29, n0), out);
+}
+
+}  // namespace m29
+}  // namespace bg
