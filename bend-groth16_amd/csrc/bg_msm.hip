@@ -40,8 +40,11 @@ static void finish_plan(MsmPlan& p, int group) {
   p.kspan = (uint32_t)p.W * p.B;
   p.nb = p.M * p.kspan;
   // buckets per level-1 group: 2, 4 or 8 (the four-role level-1 sums need m <= 8)
-  uint32_t mm = (uint32_t)tune_env("BG16_MSM_M");
-  mm = mm >= 8 || mm == 0 ? 8 : (mm >= 4 ? 4 : 2);
+  // G1's level 1 runs one wave per binary-weight role, so wide groups cost no depth; G2's
+  // runs one lane per group (a running sum of depth 2m), so its groups are narrower
+  uint32_t mm = (uint32_t)tune_env(group == 1 ? "BG16_MSM_M" : "BG16_MSM_M_G2");
+  if (mm == 0) mm = group == 1 ? 8 : 2;
+  mm = mm >= 8 ? 8 : (mm >= 4 ? 4 : 2);
   p.m = std::min<uint32_t>(mm, p.B);
   p.G = p.B / p.m;
   uint32_t kmax = group == 1 ? 256 : 128;
@@ -568,23 +571,48 @@ static void host_finish(const MsmPlan& p, const uint32_t* hout, std::vector<bgh:
   int log2M = 0;
   while ((1u << log2M) < p.m * p.K) log2M++;
   const uint32_t MW = p.M * (uint32_t)p.W;
-  std::vector<Pt> wsum(MW, Pt::inf());
-  auto one_window = [&](uint32_t w) {
+  // Window w's level-2 outputs (P_h, Q_h), h < H, give sum_h P_h + mK * sum_h h Q_h.  The
+  // H outputs are cut into T chunks of Lc (powers of two) summed in parallel: chunk j
+  // (from a = j Lc) returns S = sum P_h, U = sum Q_h and V = sum (h - a) Q_h, and
+  // sum_h h Q_h = sum_j V_j + Lc * sum_j j U_j.
+  uint32_t T = 1;
+  while (T < 16 && p.H / (2 * T) >= 16) T *= 2;
+  const uint32_t Lc = p.H / T;
+  int log2Lc = 0;
+  while ((1u << log2Lc) < Lc) log2Lc++;
+  struct Part {
+    Pt S, U, V;
+  };
+  std::vector<Part> parts((size_t)MW * T);
+  auto chunk = [&](int id) {
+    const uint32_t w = (uint32_t)id / T, j = (uint32_t)id % T;
     const uint32_t* base = hout + (uint64_t)2 * p.H * xw * w;
-    Pt sumP = Pt::inf(), run = Pt::inf(), wq = Pt::inf();
-    for (int h = (int)p.H - 1; h >= 0; h--) {
-      sumP = sumP.add(bgh::load_xyzz<E>(base + (uint64_t)xw * (2 * h)));
-      if (h > 0) {
-        run = run.add(bgh::load_xyzz<E>(base + (uint64_t)xw * (2 * h + 1)));
-        wq = wq.add(run);
+    const int a = (int)(j * Lc);
+    Pt S = Pt::inf(), run = Pt::inf(), V = Pt::inf();
+    for (int h = a + (int)Lc - 1; h >= a; h--) {
+      S = S.add(bgh::load_xyzz<E>(base + (uint64_t)xw * (2 * h)));
+      if (h > 0) run = run.add(bgh::load_xyzz<E>(base + (uint64_t)xw * (2 * h + 1)));
+      if (h > a) V = V.add(run);
+    }
+    parts[id] = {S, run, V};
+  };
+  if ((uint64_t)p.H * MW < 64)
+    for (uint32_t i = 0; i < MW * T; i++) chunk((int)i);
+  else
+    host_parallel((int)(MW * T), chunk);
+  std::vector<Pt> wsum(MW);
+  for (uint32_t w = 0; w < MW; w++) {
+    Pt S = Pt::inf(), Vs = Pt::inf(), run = Pt::inf(), jU = Pt::inf();
+    for (int j = (int)T - 1; j >= 0; j--) {
+      const Part& q = parts[(size_t)w * T + j];
+      S = S.add(q.S);
+      Vs = Vs.add(q.V);
+      if (j > 0) {
+        run = run.add(q.U);
+        jU = jU.add(run);
       }
     }
-    wsum[w] = sumP.add(wq.dbl_n(log2M));
-  };
-  if ((uint64_t)p.H * MW < 64) {
-    for (uint32_t w = 0; w < MW; w++) one_window(w);
-  } else {
-    host_parallel((int)MW, [&](int w) { one_window((uint32_t)w); });
+    wsum[w] = S.add(Vs.add(jU.dbl_n(log2Lc)).dbl_n(log2M));
   }
   res.assign(p.M, Pt::inf());
   auto horner = [&](int i) {

@@ -123,7 +123,7 @@ static unsigned nb(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t);
 static int prove_schedule() {
   static int s = [] {
     const char* e = getenv("BG16_PROVE_SCHEDULE");
-    return e ? atoi(e) : 3;
+    return e ? atoi(e) : 0;
   }();
   return s;
 }
@@ -396,11 +396,13 @@ void prove_on(bg_pk* pk, ProveSlot& sl, const uint32_t* witness, const uint32_t*
     //    accumulation and reduction over all their windows); H needs qap_h's output.
     //    Schedules (BG16_PROVE_SCHEDULE; every one gives the same proof):
     //   0: B2 on s3 and the G1 batch A, B1, L on s1 start at once; qap_h then H on s0
+    //      (default: the shortest from 2^12 to 2^18)
     //   1: B2 on s3 at once; qap_h then the G1 batch A, B1, L, H on s0
     //   2: as 0, but the G1 batch waits until B2's buckets are complete
-    //   3: as 0, but the G1 batch waits for qap_h (default: the shortest at 2^18)
+    //   3: as 0, but the G1 batch waits for qap_h
     //   4: qap_h first; then B2 on s3, the G1 batch A, B1, L on s1 and H on s0
-    //   5: as 3 on a single stream (no overlap: per-kernel times for profiling)
+    //   5: B2, qap_h, the G1 batch and H on a single stream (no overlap: per-kernel times
+    //      for profiling)
     for (hipStream_t sx : {s1, s2, s3}) BG_HIP(hipStreamWaitEvent(sx, ev[2], 0));
     const uint32_t* g1p[4] = {pk->pa.as<uint32_t>(), pk->pb1.as<uint32_t>(), pk->pl.as<uint32_t>(),
                               pk->ph.as<uint32_t>()};
