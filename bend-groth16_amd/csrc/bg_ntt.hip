@@ -28,6 +28,20 @@ __global__ void k_geom(uint32_t* __restrict__ out, uint64_t n, const uint32_t* _
   store_fp(out + 8 * i, x);
 }
 
+// per-stage twiddles: out[n - n/2^k + j] = tw[j << k] for j < n/2^(k+1), k < log2
+__global__ void k_stage_twiddles(const uint32_t* __restrict__ tw, uint32_t* __restrict__ out, int log2) {
+  const uint64_t n = 1ull << log2;
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  uint64_t j = i;
+  int k = 0;
+  while (j >= (n >> (k + 1))) {
+    j -= n >> (k + 1);
+    k++;
+  }
+  store_fp(out + 8 * i, load_fp(tw + 8 * (j << k)));
+}
+
 __device__ __forceinline__ uint32_t bitrev(uint32_t i, int log2) { return log2 ? (__brev(i) >> (32 - log2)) : 0; }
 
 // out[k] = in[rev(k)] * (scale ? scale[k] : c)   (bit-reversed order -> natural order)
@@ -50,8 +64,9 @@ __global__ void k_gather(const uint32_t* __restrict__ in, uint32_t* __restrict__
 // elements per row keep the global accesses coalesced).
 //   DIF: k ascending,  (u, v) -> (u + v, (u - v) w)
 //   DIT: k descending, (u, v) -> (u + v w, u - v w)
-// with w = omega^((i mod m) * 2^k), i the upper element's index, from the n/2-entry
-// table tw.  scale (optional) multiplies each element by scale[index] as it is loaded.
+// with w = omega^((i mod m) * 2^k), i the upper element's index, read from stage k's part
+// of the per-stage table tw.  scale (optional) multiplies each element by scale[index] as
+// it is loaded.
 // src and dst may alias (in-place pass).
 template <bool DIT>
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass(const uint32_t* src, uint32_t* dst, int log2, int k0, int t,
@@ -81,7 +96,9 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass(const uint32_t* src, u
       uint32_t jl = bj & ((1u << lmr) - 1);
       uint32_t j0 = ((bj >> lmr) << (lmr + 1)) + jl;
       uint32_t e0 = (j0 << logC) + c, e1 = e0 + (1u << (lmr + logC));
-      uint64_t ex = ((((uint64_t)jl) << S_log) + q0 + c) << k;
+      // stage k's table holds omega^(j 2^k) contiguously from n - n/2^k, so the lanes of a
+      // row (consecutive c) read consecutive twiddles
+      uint64_t ex = ((1ull << log2) - (1ull << (log2 - k))) + (((uint64_t)jl) << S_log) + q0 + c;
       Fp w = load_fp(tw + 8 * ex);
       Fp u = sh[e0], v = sh[e1];
       if (DIT) {
@@ -169,8 +186,15 @@ NttDomain& ntt_domain(int log2) {
   bgh::Fr w = omega_of(log2, false), wi = omega_of(log2, true);
   bgh::Fr g = fr_from_u64(5), gi = g.inv();
   bgh::Fr ninv = fr_from_u64(n).inv();
-  build_geom(dom->tw_fwd, n / 2, w, bgh::Fr::one(), false, s);
-  build_geom(dom->tw_inv, n / 2, wi, bgh::Fr::one(), false, s);
+  for (int dir = 0; dir < 2; dir++) {
+    DevArray flat;  // omega^j, j < n/2
+    build_geom(flat, n / 2, dir ? wi : w, bgh::Fr::one(), false, s);
+    DevArray& st = dir ? dom->tw_inv : dom->tw_fwd;
+    st.alloc(n * 32);
+    k_stage_twiddles<<<nblk(n, 256), 256, 0, s>>>(flat.as<uint32_t>(), st.as<uint32_t>(), log2);
+    BG_HIP(hipGetLastError());
+    BG_HIP(hipStreamSynchronize(s));  // flat is freed at the end of this scope
+  }
   build_geom(dom->sc_g, n, g, bgh::Fr::one(), false, s);
   build_geom(dom->sc_ninv_g, n, g, ninv, false, s);
   build_geom(dom->sc_ninv_gi, n, gi, ninv, false, s);

@@ -2,8 +2,8 @@
 //
 // Replaces Ntt.fft / ifft / coset_fft / coset_ifft and Ntt.qap_h (bend2/src/ntt.bend:
 // 431-459).  Natural order in and out.  Per domain size the library keeps twiddle and
-// scale tables resident in HBM (omega^j, n^-1 g^j, n^-1 g^-j), built on the device from
-// two short host-computed geometric tables.
+// scale tables resident in HBM (per-stage omega^(j 2^k), n^-1 g^j, n^-1 g^-j), built on the
+// device from two short host-computed geometric tables.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -14,8 +14,8 @@ namespace bg {
 struct NttDomain {
   int log2 = 0;
   uint64_t n = 0;
-  DevArray tw_fwd;       // omega^j, j < n/2 (Montgomery)
-  DevArray tw_inv;       // omega^-j
+  DevArray tw_fwd;       // per-stage twiddles: stage k's omega^(j 2^k), j < n/2^(k+1), at n - n/2^k
+  DevArray tw_inv;       // the same for omega^-1
   DevArray sc_g;         // g^j (Montgomery)                    -- coset_fft input scale
   DevArray sc_ninv_g;    // n^-1 g^j (Montgomery)               -- ifft then coset_fft
   DevArray sc_ninv_gi;   // n^-1 g^-j (Montgomery)              -- coset_ifft output scale
