@@ -14,7 +14,8 @@ namespace bg {
 struct ProveSlot {
   hipStream_t s[4] = {};
   hipEvent_t ev[16] = {};
-  DevArray z, zc, zraw, ea, eb, ec, h, tmp;
+  DevArray z, zc, ea, eb, ec, h, tmp;
+  HostBuf zpin;  // pinned staging of a host witness (packed words)
   uint64_t cap_nz = 0, cap_d = 0;
   MsmWork work[3];  // G1 batch (A, B1, L), H, B2
   ProveSlot();
