@@ -3,17 +3,21 @@
 // bg_prove_batch).
 //
 // Requests (key, witness, r/s) queue in arrival order.  Each worker thread owns one
-// ProveSlot (four streams and the work arrays of one proof) and takes the oldest request
-// whose work arrays fit the device-memory budget, so proofs of different sizes run
-// concurrently: one proof's latency-bound reductions and host finishing overlap the
-// others' accumulation.  Keys stay resident (bg_pk_load) and are shared read-only by all
-// slots.  Every proof is computed exactly as bg_prove computes it alone.
+// ProveSlot (four streams and the work arrays of its proofs) and takes the oldest request
+// whose work arrays fit the device-memory budget, together with up to batch_limit - 1
+// later requests whose keys have the same shape (prove_batch_on: one launch sequence for
+// all of them, so small proofs share their latency-bound reductions and launches).
+// Proofs of different sizes run concurrently on different slots: one proof's
+// latency-bound reductions and host finishing overlap the others' accumulation.  Keys stay
+// resident (bg_pk_load) and are shared read-only by all slots.  Every proof is computed
+// exactly as bg_prove computes it alone.
 //
 // The reference proves one instance per process run (bend2/prove.bend:129-162); this is
 // the serving layer the batched and served configurations run on.
 #include "bg_prover.hpp"
 #include <chrono>
 #include <condition_variable>
+#include <algorithm>
 #include <deque>
 #include <map>
 #include <memory>
@@ -51,11 +55,35 @@ struct bg_engine {
   std::vector<std::thread> workers;
   std::vector<uint64_t> reserved;  // per slot: device bytes its work arrays may hold
   uint64_t budget = 0, next_ticket = 1, completed = 0;
+  int running = 0;  // slots proving now
   bool stop = false;
   int device = 0;
 };
 
 namespace bg {
+
+// Proofs of one shape a slot takes together: 16 up to 2^13 constraints (domain 2^14), 8 up
+// to 2^16, halving with each size step above (a batch completes all at once, so large
+// proofs stay alone).  BG16_ENGINE_BATCH caps it (1 disables batching).
+static size_t batch_limit(bg_pk* pk) {
+  uint32_t hdr[8];
+  if (bg_pk_header(pk, hdr) != 0) return 1;
+  const uint32_t lg = hdr[3];  // log2 of the domain: constraints + 2 rounded up
+  size_t lim = lg <= 14 ? 16 : lg <= 17 ? 8 : (lg >= 20 ? 1 : (size_t)1 << (20 - lg));
+  const char* env = getenv("BG16_ENGINE_BATCH");
+  if (env && atoi(env) > 0) lim = std::min<size_t>(lim, (size_t)atoi(env));
+  return lim;
+}
+
+// Longest a partial batch waits for more requests of its shape while the device is busy
+// (BG16_ENGINE_WAIT_US, default 2000 us).
+static int64_t batch_wait_us() {
+  static const int64_t w = [] {
+    const char* e = getenv("BG16_ENGINE_WAIT_US");
+    return e ? (int64_t)atoll(e) : (int64_t)2000;
+  }();
+  return w;
+}
 
 static uint64_t total_reserved(const bg_engine* e) {
   uint64_t t = 0;
@@ -70,30 +98,48 @@ static void worker_main(bg_engine* e, int slot_id) {
   } catch (const Error&) {
   }
   for (;;) {
-    Request* r = nullptr;
+    std::vector<Request*> group;
     {
       std::unique_lock<std::mutex> lk(e->mu);
       for (;;) {
         if (e->stop) return;
         if (!e->queue.empty()) {
           Request* head = e->queue.front();
+          // the head and the next queued requests of the same key shape
+          std::vector<Request*> cand = {head};
+          const size_t lim = batch_limit(head->pk);
+          for (auto it = e->queue.begin() + 1; it != e->queue.end() && cand.size() < lim; ++it)
+            if (prove_batchable(head->pk, (*it)->pk)) cand.push_back(*it);
+          // a partial batch waits for more requests while another slot keeps the device busy,
+          // up to max_wait after the head arrived (a full batch, or an idle device, goes now)
+          const auto max_wait = std::chrono::microseconds(batch_wait_us());
+          const auto ready_at = head->t_submit + max_wait;
+          if (cand.size() < lim && e->running > 0 && Clock::now() < ready_at) {
+            e->cv_work.wait_until(lk, ready_at);
+            continue;
+          }
           uint64_t mine = e->reserved[slot_id];
           uint64_t others = total_reserved(e) - mine;
-          uint64_t grown = std::max(mine, head->need);
-          if (others + grown <= e->budget) {
-            e->queue.pop_front();
-            e->reserved[slot_id] = grown;
-            r = head;
-            break;
+          uint64_t need = head->need * cand.size();
+          if (cand.size() > 1 && others + std::max(mine, need) > e->budget) {
+            cand.resize(1);  // the batch does not fit: the head alone
+            need = head->need;
           }
-          if (mine && others + head->need <= e->budget) {
-            // the head fits once this idle slot gives its arrays back
-            e->queue.pop_front();
-            e->reserved[slot_id] = head->need;
-            r = head;
-            lk.unlock();
-            if (slot) slot->release_buffers();
-            lk.lock();
+          uint64_t grown = std::max(mine, need);
+          const bool fits = others + grown <= e->budget;
+          const bool fits_after_release = !fits && mine && others + need <= e->budget;
+          if (fits || fits_after_release) {
+            // claim the requests before the lock may be dropped
+            for (Request* r : cand) e->queue.erase(std::find(e->queue.begin(), e->queue.end(), r));
+            group = cand;
+            e->running++;
+            e->reserved[slot_id] = fits ? grown : need;
+            if (fits_after_release) {
+              // the batch fits once this idle slot gives its arrays back
+              lk.unlock();
+              if (slot) slot->release_buffers();
+              lk.lock();
+            }
             break;
           }
           if (mine) {
@@ -109,7 +155,7 @@ static void worker_main(bg_engine* e, int slot_id) {
         e->cv_work.wait(lk);
       }
     }
-    r->t_start = Clock::now();
+    const auto t_start = Clock::now();
     int rc = 0;
     std::string msg;
     if (!slot) {
@@ -117,17 +163,30 @@ static void worker_main(bg_engine* e, int slot_id) {
       msg = "engine worker could not create its streams";
     } else {
       rc = guard([&] {
-        prove_on(r->pk, *slot, r->dev_witness ? r->dev_witness : r->witness.data(), r->rs, r->proof, r->flags,
-                 nullptr);
+        if (group.size() == 1) {
+          Request* r = group[0];
+          prove_on(r->pk, *slot, r->dev_witness ? r->dev_witness : r->witness.data(), r->rs, r->proof, r->flags,
+                   nullptr);
+        } else {
+          std::vector<ProveJob> jobs;
+          for (Request* r : group)
+            jobs.push_back({r->pk, r->dev_witness ? r->dev_witness : r->witness.data(), r->rs, r->proof, r->flags});
+          prove_batch_on(jobs.data(), (int)jobs.size(), *slot);
+        }
       });
       if (rc < 0) msg = bg_last_error();
     }
-    r->t_end = Clock::now();
+    const auto t_end = Clock::now();
     {
       std::lock_guard<std::mutex> lk(e->mu);
-      r->status = rc < 0 ? rc : 0;
-      r->err = msg;
-      e->completed++;
+      e->running--;
+      for (Request* r : group) {
+        r->t_start = t_start;
+        r->t_end = t_end;
+        r->status = rc < 0 ? rc : 0;
+        r->err = msg;
+        e->completed++;
+      }
       // actual footprint after the run (grow-only buffers)
       if (slot) e->reserved[slot_id] = std::max(e->reserved[slot_id], slot->device_bytes());
     }

@@ -45,10 +45,13 @@ __global__ void k_stage_twiddles(const uint32_t* __restrict__ tw, uint32_t* __re
 __device__ __forceinline__ uint32_t bitrev(uint32_t i, int log2) { return log2 ? (__brev(i) >> (32 - log2)) : 0; }
 
 // out[k] = in[rev(k)] * (scale ? scale[k] : c)   (bit-reversed order -> natural order)
+// blockIdx.y: transform of a batch of contiguous 2^log2-element arrays
 __global__ void k_gather(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, int log2,
                          const uint32_t* __restrict__ scale, Fp c, int use_c) {
   uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= (1u << log2)) return;
+  in += 8 * ((uint64_t)blockIdx.y << log2);
+  out += 8 * ((uint64_t)blockIdx.y << log2);
   Fp x = load_fp(in + 8ull * bitrev(k, log2));
   if (scale)
     x = Fr::mul(x, load_fp(scale + 8ull * k));
@@ -61,7 +64,8 @@ __global__ void k_gather(const uint32_t* __restrict__ in, uint32_t* __restrict__
 // only couple elements whose indices differ by multiples of S = 2^(log2-k0-t) inside an
 // aligned block of 2^(log2-k0) elements, so a workgroup takes the 2^t elements
 // base + j*S + q (j < 2^t) for each of C = 2^logC consecutive offsets q (C consecutive
-// elements per row keep the global accesses coalesced).
+// elements per row keep the global accesses coalesced).  blockIdx.y selects the transform
+// of a batch of contiguous 2^log2-element arrays (the tables are shared).
 //   DIF: k ascending,  (u, v) -> (u + v, (u - v) w)
 //   DIT: k descending, (u, v) -> (u + v w, u - v w)
 // with w = omega^((i mod m) * 2^k), i the upper element's index, read from stage k's part
@@ -73,6 +77,8 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass(const uint32_t* src, u
                                                           int logC, const uint32_t* __restrict__ tw,
                                                           const uint32_t* __restrict__ scale) {
   __shared__ Fp sh[1 << NTT_TILE_LOG2];
+  src += 8 * ((uint64_t)blockIdx.y << log2);
+  dst += 8 * ((uint64_t)blockIdx.y << log2);
   const int S_log = log2 - k0 - t;
   const uint32_t C = 1u << logC;
   const uint32_t qgroups = 1u << (S_log - logC);
@@ -244,9 +250,10 @@ static std::vector<NttPass> ntt_plan(int log2) {
   return out;
 }
 
+// batch: number of contiguous 2^log2-element transforms (grid y)
 static void launch_pass(bool dit, const uint32_t* src, uint32_t* dst, int log2, const NttPass& p,
-                        const uint32_t* tw, const uint32_t* scale, hipStream_t s) {
-  unsigned wgs = 1u << (log2 - p.t - p.logC);
+                        const uint32_t* tw, const uint32_t* scale, hipStream_t s, unsigned batch) {
+  const dim3 wgs(1u << (log2 - p.t - p.logC), batch);
   if (dit)
     k_ntt_pass<true><<<wgs, NTT_THREADS, 0, s>>>(src, dst, log2, p.k0, p.t, p.logC, tw, scale);
   else
@@ -256,26 +263,26 @@ static void launch_pass(bool dit, const uint32_t* src, uint32_t* dst, int log2, 
 
 // natural order in -> bit-reversed order out; scale multiplies the (natural) input
 static void dif(const uint32_t* src, uint32_t* dst, int log2, const uint32_t* tw, const uint32_t* scale,
-                hipStream_t s) {
+                hipStream_t s, unsigned batch = 1) {
   auto plan = ntt_plan(log2);
   for (size_t i = 0; i < plan.size(); i++)
-    launch_pass(false, i == 0 ? src : dst, dst, log2, plan[i], tw, i == 0 ? scale : nullptr, s);
+    launch_pass(false, i == 0 ? src : dst, dst, log2, plan[i], tw, i == 0 ? scale : nullptr, s, batch);
 }
 
 // bit-reversed order in -> natural order out; scale multiplies the (bit-reversed) input
 static void dit(const uint32_t* src, uint32_t* dst, int log2, const uint32_t* tw, const uint32_t* scale,
-                hipStream_t s) {
+                hipStream_t s, unsigned batch = 1) {
   auto plan = ntt_plan(log2);
   for (size_t i = plan.size(); i-- > 0;)
     launch_pass(true, i + 1 == plan.size() ? src : dst, dst, log2, plan[i], tw,
-                i + 1 == plan.size() ? scale : nullptr, s);
+                i + 1 == plan.size() ? scale : nullptr, s, batch);
 }
 
 static void gather(const uint32_t* in, uint32_t* out, int log2, const uint32_t* scale, const uint32_t* c,
-                   hipStream_t s) {
+                   hipStream_t s, unsigned batch = 1) {
   Fp cf{};
   if (c) cf = to_fp(c);
-  k_gather<<<nblk(1ull << log2, 256), 256, 0, s>>>(in, out, log2, scale, cf, c ? 1 : 0);
+  k_gather<<<dim3(nblk(1ull << log2, 256), batch), 256, 0, s>>>(in, out, log2, scale, cf, c ? 1 : 0);
   BG_HIP(hipGetLastError());
 }
 
@@ -293,20 +300,21 @@ void ntt_run(uint32_t* d, uint32_t* tmp, int log2, int direction, hipStream_t s)
 }
 
 void qap_h_run(uint32_t* a, uint32_t* b, uint32_t* c, uint32_t* h, uint32_t* tmp, int log2, bool canonical_out,
-               hipStream_t s) {
+               hipStream_t s, unsigned batch) {
   (void)tmp;
   NttDomain& D = ntt_domain(log2);
   uint64_t n = D.n;
   uint32_t* v3[3] = {a, b, c};
   for (uint32_t* v : v3) {
     // coefficients (bit-reversed, times n) -> n^-1 g^k scale on load -> coset evaluations
-    dif(v, v, log2, D.tw_inv.as<uint32_t>(), nullptr, s);
-    dit(v, v, log2, D.tw_fwd.as<uint32_t>(), D.sc_ninv_g_rev.as<uint32_t>(), s);
+    dif(v, v, log2, D.tw_inv.as<uint32_t>(), nullptr, s, batch);
+    dit(v, v, log2, D.tw_fwd.as<uint32_t>(), D.sc_ninv_g_rev.as<uint32_t>(), s, batch);
   }
-  k_qap_point<<<nblk(n, 256), 256, 0, s>>>(a, b, c, n, to_fp(D.zinv));
+  k_qap_point<<<nblk(n * batch, 256), 256, 0, s>>>(a, b, c, n * batch, to_fp(D.zinv));
   BG_HIP(hipGetLastError());
-  dif(a, a, log2, D.tw_inv.as<uint32_t>(), nullptr, s);
-  gather(a, h, log2, canonical_out ? D.sc_ninv_gi_c.as<uint32_t>() : D.sc_ninv_gi.as<uint32_t>(), nullptr, s);
+  dif(a, a, log2, D.tw_inv.as<uint32_t>(), nullptr, s, batch);
+  gather(a, h, log2, canonical_out ? D.sc_ninv_gi_c.as<uint32_t>() : D.sc_ninv_gi.as<uint32_t>(), nullptr, s,
+         batch);
 }
 
 }  // namespace bg

@@ -32,8 +32,9 @@ NttDomain& ntt_domain(int log2);
 void ntt_run(uint32_t* d, uint32_t* tmp, int log2, int direction, hipStream_t s);
 
 // qap_h on device buffers a, b, c (clobbered); h receives n coefficients, canonical if
-// canonical_out (the MSM scalar form, bend2/prove.bend:56), else Montgomery.
+// canonical_out (the MSM scalar form, bend2/prove.bend:56), else Montgomery.  batch > 1:
+// a, b, c, h each hold that many contiguous n-element instances, transformed together.
 void qap_h_run(uint32_t* a, uint32_t* b, uint32_t* c, uint32_t* h, uint32_t* tmp, int log2, bool canonical_out,
-               hipStream_t s);
+               hipStream_t s, unsigned batch = 1);
 
 }  // namespace bg

@@ -344,6 +344,30 @@ extern "C" int bg_prove(bg_pk* pk, const uint32_t* witness, const uint32_t* rs, 
 
 namespace bg {
 
+// Witness of nz packed elements into device memory z on stream s.  A host witness is
+// packed (or copied) into the pinned staging buffer zp by the host worker pool, then
+// moved by one DMA: a pageable source goes through the runtime's staged copy (~2.6 GB/s
+// for the 16.8 MB words16 witness of a 2^18 proof).  zp must stay untouched until the
+// copy has run.
+static void upload_witness(const uint32_t* witness, int flags, uint64_t nz, uint32_t* z, uint32_t* zp,
+                           hipStream_t s) {
+  const bool words16 = (flags & 1) == BG_FMT_WORDS16;
+  if (flags & BG_MEM_DEVICE) {
+    BG_HIP(hipMemcpyAsync(z, witness, 32 * nz, hipMemcpyDeviceToDevice, s));
+    return;
+  }
+  const uint64_t chunk = 1ull << 14;
+  host_parallel((int)((nz + chunk - 1) / chunk), [&](int t) {
+    const uint64_t lo = (uint64_t)t * chunk, hi = std::min<uint64_t>(nz, lo + chunk);
+    if (words16) {
+      for (uint64_t i = 8 * lo; i < 8 * hi; i++) zp[i] = witness[2 * i] | (witness[2 * i + 1] << 16);
+    } else {
+      memcpy(zp + 8 * lo, witness + 8 * lo, 32 * (hi - lo));
+    }
+  });
+  BG_HIP(hipMemcpyAsync(z, zp, 32 * nz, hipMemcpyHostToDevice, s));
+}
+
 void prove_on(bg_pk* pk, ProveSlot& sl, const uint32_t* witness, const uint32_t* rs, uint32_t* proof_out, int flags,
               double* t_ms) {
   {
@@ -368,24 +392,7 @@ void prove_on(bg_pk* pk, ProveSlot& sl, const uint32_t* witness, const uint32_t*
     BG_HIP(hipEventRecord(ev[0], s0));
     // 1. witness upload (16-bit words are packed on the device)
     uint32_t* z = sl.z.as<uint32_t>();
-    if (dev_witness) {
-      BG_HIP(hipMemcpyAsync(z, witness, 32 * nz, hipMemcpyDeviceToDevice, s0));
-    } else {
-      // host witness: packed (or copied) into the slot's pinned staging buffer by the host
-      // worker pool, then one DMA (a pageable source goes through the runtime's staged
-      // copy, ~2.6 GB/s for the 16.8 MB words16 witness of a 2^18 proof)
-      uint32_t* zp = static_cast<uint32_t*>(sl.zpin.get(32 * nz));
-      const uint64_t chunk = 1ull << 14;
-      host_parallel((int)((nz + chunk - 1) / chunk), [&](int t) {
-        const uint64_t lo = (uint64_t)t * chunk, hi = std::min<uint64_t>(nz, lo + chunk);
-        if (words16) {
-          for (uint64_t i = 8 * lo; i < 8 * hi; i++) zp[i] = witness[2 * i] | (witness[2 * i + 1] << 16);
-        } else {
-          memcpy(zp + 8 * lo, witness + 8 * lo, 32 * (hi - lo));
-        }
-      });
-      BG_HIP(hipMemcpyAsync(z, zp, 32 * nz, hipMemcpyHostToDevice, s0));
-    }
+    upload_witness(witness, flags, nz, z, static_cast<uint32_t*>(sl.zpin.get(32 * nz)), s0);
     BG_HIP(hipEventRecord(ev[1], s0));
     // 2. evaluation vectors and canonical scalars
     uint32_t* zc = sl.zc.as<uint32_t>();
@@ -548,6 +555,140 @@ void prove_on(bg_pk* pk, ProveSlot& sl, const uint32_t* witness, const uint32_t*
       t_ms[8] = std::chrono::duration<double, std::milli>(t1 - t0).count();
     }
   }
+}
+
+bool prove_batchable(const bg_pk* a, const bg_pk* b) {
+  return a && b && a->fixed && b->fixed && memcmp(a->hdr, b->hdr, sizeof a->hdr) == 0 && a->c_g1 == b->c_g1 &&
+         a->c_h == b->c_h && a->c_b2 == b->c_b2;
+}
+
+void prove_batch_on(const ProveJob* jobs, int P, ProveSlot& sl) {
+  BG_REQUIRE(P >= 1 && jobs, BG_ERR_ARG, "prove batch: no jobs");
+  const bg_pk* pk0 = jobs[0].pk;
+  for (int j = 0; j < P; j++) {
+    const ProveJob& q = jobs[j];
+    BG_REQUIRE(q.pk && q.witness && q.rs && q.proof_out, BG_ERR_ARG, "prove batch: null argument");
+    BG_REQUIRE(!((q.flags & BG_MEM_DEVICE) && (q.flags & 1) == BG_FMT_WORDS16), BG_ERR_ARG,
+               "prove batch: a device witness must be packed");
+    BG_REQUIRE(prove_batchable(pk0, q.pk), BG_ERR_ARG, "prove batch: keys of different shapes");
+  }
+  (void)ctx();
+  const uint32_t nc = pk0->hdr[0], ni = pk0->hdr[1], nw = pk0->hdr[2], lg = pk0->hdr[3];
+  const uint32_t na = pk0->hdr[4], nbq = pk0->hdr[5], nh = pk0->hdr[6], nl = pk0->hdr[7];
+  const uint64_t d = 1ull << lg, nz = (uint64_t)ni + nw;
+  hipStream_t s0 = sl.s[0], s1 = sl.s[1], s3 = sl.s[3];
+  hipEvent_t* ev = sl.ev;
+  // every per-proof array holds the P proofs back to back
+  sl.ensure(P * nz, P * d);
+  uint32_t *z = sl.z.as<uint32_t>(), *zc = sl.zc.as<uint32_t>(), *ea = sl.ea.as<uint32_t>(),
+           *eb = sl.eb.as<uint32_t>(), *ec = sl.ec.as<uint32_t>(), *h = sl.h.as<uint32_t>();
+  uint32_t* zp = static_cast<uint32_t*>(sl.zpin.get(32 * nz * P));
+  for (int j = 0; j < P; j++) upload_witness(jobs[j].witness, jobs[j].flags, nz, z + 8 * nz * j, zp + 8 * nz * j, s0);
+  {
+    ScopedKTime kt("evals", s0);
+    k_from_mont<<<nb(P * nz, 256), 256, 0, s0>>>(z, zc, P * nz);
+    BG_HIP(hipEventRecord(ev[2], s0));  // canonical scalars of every proof ready
+    for (int j = 0; j < P; j++) {
+      const bg_pk* pk = jobs[j].pk;
+      const uint32_t* zj = z + 8 * nz * j;
+      k_evals<<<nb(d, 256), 256, 0, s0>>>(pk->A.offs.as<uint32_t>(), pk->A.cols.as<uint32_t>(),
+                                          pk->A.coef.as<uint32_t>(), zj, nc, ni, 1, d, ea + 8 * d * j);
+      k_evals<<<nb(d, 256), 256, 0, s0>>>(pk->B.offs.as<uint32_t>(), pk->B.cols.as<uint32_t>(),
+                                          pk->B.coef.as<uint32_t>(), zj, nc, ni, 0, d, eb + 8 * d * j);
+      k_evals<<<nb(d, 256), 256, 0, s0>>>(pk->C.offs.as<uint32_t>(), pk->C.cols.as<uint32_t>(),
+                                          pk->C.coef.as<uint32_t>(), zj, nc, ni, 0, d, ec + 8 * d * j);
+    }
+    BG_HIP(hipGetLastError());
+  }
+  // MSMs as schedule 0, each over all P proofs: B2 (P MSMs) on s3 and the G1 batch (A, B1,
+  // L of every proof: 3P MSMs) on s1 at once; qap_h of all proofs, then H (P MSMs) on s0
+  std::vector<const uint32_t*> tb2(P), sb2(P), tg1(3 * P), sg1(3 * P), th(P), sh(P);
+  std::vector<uint64_t> nb2(P, nbq), ng1(3 * P), nhv(P, nh);
+  for (int j = 0; j < P; j++) {
+    const bg_pk* pk = jobs[j].pk;
+    const uint32_t* zcj = zc + 8 * nz * j;
+    tb2[j] = pk->pb2.as<uint32_t>();
+    sb2[j] = zcj;
+    tg1[3 * j] = pk->pa.as<uint32_t>();
+    tg1[3 * j + 1] = pk->pb1.as<uint32_t>();
+    tg1[3 * j + 2] = pk->pl.as<uint32_t>();
+    sg1[3 * j] = zcj;
+    sg1[3 * j + 1] = zcj;
+    sg1[3 * j + 2] = zcj + 8 * ni;
+    ng1[3 * j] = na;
+    ng1[3 * j + 1] = nbq;
+    ng1[3 * j + 2] = nl;
+    th[j] = pk->ph.as<uint32_t>();
+    sh[j] = h + 8 * d * j;
+  }
+  for (hipStream_t sx : {s1, s3}) BG_HIP(hipStreamWaitEvent(sx, ev[2], 0));
+  msm_enqueue_fixed(2, P, tb2.data(), sb2.data(), nb2.data(), pk0->c_b2, s3, sl.work[2]);
+  msm_enqueue_fixed(1, 3 * P, tg1.data(), sg1.data(), ng1.data(), pk0->c_g1, s1, sl.work[0]);
+  {
+    ScopedKTime kt("qap_h", s0);
+    qap_h_run(ea, eb, ec, h, sl.tmp.as<uint32_t>(), (int)lg, true, s0, (unsigned)P);
+  }
+  msm_enqueue_fixed(1, P, th.data(), sh.data(), nhv.data(), pk0->c_h, s0, sl.work[1]);
+
+  // host: per proof r, s scalars and the delta / alpha / beta terms while the device runs
+  struct Comb {
+    uint64_t rc[4], sc[4], rsc[4];
+    bgh::P1 alpha, beta1, r_d1, s_d1, rs_d1, A, C0;
+    bgh::P2 beta2, s_d2;
+    uint32_t proof[64];
+  };
+  std::vector<Comb> cb(P);
+  host_parallel(P, [&](int j) {
+    const ProveJob& q = jobs[j];
+    Comb& c = cb[j];
+    uint32_t rs_packed[16];
+    if ((q.flags & 1) == BG_FMT_WORDS16)
+      pack_words(q.rs, rs_packed, 16, "rs");
+    else
+      memcpy(rs_packed, q.rs, sizeof rs_packed);
+    canon_words(rs_packed, c.rc);
+    canon_words(rs_packed + 8, c.sc);
+    bgh::Fr x = (bgh::Fr::from_u32(rs_packed) * bgh::Fr::from_u32(rs_packed + 8)).from_mont();
+    memcpy(c.rsc, x.v, 32);
+    const bg_pk* pk = q.pk;
+    c.alpha = bgh::load_affine<bgh::Fq>(pk->g1);
+    c.beta1 = bgh::load_affine<bgh::Fq>(pk->g1 + 16);
+    const bgh::P1 delta1 = bgh::load_affine<bgh::Fq>(pk->g1 + 32);
+    c.beta2 = bgh::load_affine<bgh::Fq2>(pk->g2);
+    const bgh::P2 delta2 = bgh::load_affine<bgh::Fq2>(pk->g2 + 32);
+    c.r_d1 = delta1.mul(c.rc);
+    c.s_d1 = delta1.mul(c.sc);
+    c.rs_d1 = delta1.mul(c.rsc);
+    c.s_d2 = delta2.mul(c.sc);
+  });
+  // combination (groth16.bend:298-308) as in prove_on, the H term last
+  std::vector<uint32_t> g1res(3 * P * 16), b2res(P * 32), hres(P * 16);
+  msm_finish(sl.work[0], g1res.data());
+  host_parallel(P, [&](int j) {
+    Comb& c = cb[j];
+    const uint32_t* r = &g1res[48 * j];
+    c.A = bgh::load_affine<bgh::Fq>(r).add(c.r_d1).add(c.alpha);
+    const bgh::P1 B1 = bgh::load_affine<bgh::Fq>(r + 16).add(c.s_d1).add(c.beta1);
+    c.C0 = c.A.mul(c.sc).add(B1.mul(c.rc)).add(c.rs_d1.neg()).add(bgh::load_affine<bgh::Fq>(r + 32));
+    bgh::store_affine(c.A, c.proof);
+  });
+  msm_finish(sl.work[2], b2res.data());
+  msm_finish(sl.work[1], hres.data());
+  host_parallel(P, [&](int j) {
+    Comb& c = cb[j];
+    const bgh::P2 B2 = bgh::load_affine<bgh::Fq2>(&b2res[32 * j]).add(c.s_d2).add(c.beta2);
+    bgh::store_affine(B2, c.proof + 16);
+    bgh::store_affine(c.C0.add(bgh::load_affine<bgh::Fq>(&hres[16 * j])), c.proof + 48);
+    uint32_t* out = jobs[j].proof_out;
+    if ((jobs[j].flags & 1) == BG_FMT_WORDS16) {
+      for (int i = 0; i < 64; i++) {
+        out[2 * i] = c.proof[i] & 0xFFFFu;
+        out[2 * i + 1] = c.proof[i] >> 16;
+      }
+    } else {
+      memcpy(out, c.proof, sizeof c.proof);
+    }
+  });
 }
 
 }  // namespace bg

@@ -34,4 +34,20 @@ uint64_t prove_bytes_estimate(const bg_pk* pk);
 void prove_on(bg_pk* pk, ProveSlot& sl, const uint32_t* witness, const uint32_t* rs, uint32_t* proof_out, int flags,
               double* t_ms);
 
+// One proof of a batch: key, witness, r/s and output as bg_prove's arguments.
+struct ProveJob {
+  bg_pk* pk;
+  const uint32_t* witness;
+  const uint32_t* rs;
+  uint32_t* proof_out;
+  int flags;
+};
+// Keys whose proofs can share launches: fixed-base tables, identical headers and widths.
+bool prove_batchable(const bg_pk* a, const bg_pk* b);
+// Runs P proofs of batchable keys together on slot sl: one upload pass, one qap_h launch
+// sequence over all P domains, and the five MSMs of every proof as three batched MSMs
+// (B2 x P, the G1 batch A/B1/L x P, H x P).  Each proof equals the proof prove_on makes
+// alone.  Blocks until every proof is on the host.
+void prove_batch_on(const ProveJob* jobs, int P, ProveSlot& sl);
+
 }  // namespace bg

@@ -2,7 +2,9 @@
 mix of concurrent requests of different sizes equals the proof of the same inputs made
 alone by bg_prove (itself pinned to the oracle prover and the trapdoor closed form in
 test_gpu_prove.py), bit for bit; a device-memory budget that admits one proof at a time
-changes nothing but the schedule."""
+changes nothing but the schedule.  Queued requests of one key shape are proved together
+(one launch sequence for up to 8 proofs, keys may differ); those batches are checked the
+same way, including batches mixing two keys and both witness encodings."""
 import numpy as np
 import pytest
 
@@ -27,6 +29,10 @@ def keys(tmp_path_factory):
         d = tmp_path_factory.mktemp(f"e{lg}")
         bendg.setup_square_chain(d, lg, seed=900 + lg)
         out[lg] = d
+    # a second key of the K=10 shape (own trapdoor and witness): batches mix keys
+    d = tmp_path_factory.mktemp("e10b")
+    bendg.setup_square_chain(d, 10, seed=77)
+    out["10b"] = d
     return out
 
 
@@ -109,3 +115,28 @@ def test_prove_batch_packed(loaded, keys):
     outs = bendg.prove_batch([pks[lg] for lg in order], ws, rr, fmt="packed", slots=3)
     for lg, o in zip(order, outs):
         assert np.array_equal(o, bendg.pack16(solo[lg].reshape(-1, 16)).reshape(-1))
+
+
+@pytest.mark.parametrize("slots", [1, 2])
+def test_engine_batches_two_keys_of_one_shape(loaded, keys, slots):
+    """16 queued K=10 requests over two keys (words16 and packed witnesses interleaved) and
+    a K=12 request between them: the slots take them in batches of one shape; every
+    proof equals its solo proof."""
+    pks, solo = loaded
+    eng = bendg.Engine(slots=slots)
+    order = ["10b", 10] * 4 + [12] + [10, "10b"] * 4
+    tickets = []
+    for i, k in enumerate(order):
+        z, rs = _files(keys[k])
+        if i % 3 == 0:
+            zp = bendg.pack16(z.reshape(-1, 16)).reshape(-1)
+            rp = bendg.pack16(rs.reshape(-1, 16)).reshape(-1)
+            tickets.append((k, "packed", eng.submit(pks[k], zp, rp, fmt="packed")))
+        else:
+            tickets.append((k, "words16", eng.submit(pks[k], z, rs, fmt="words16")))
+    print(f"engine batch slots={slots} order={order}", flush=True)
+    for k, fmt, t in tickets:
+        proof, _ = eng.wait(t)
+        want = solo[k] if fmt == "words16" else bendg.pack16(solo[k].reshape(-1, 16)).reshape(-1)
+        assert np.array_equal(proof, want), f"key {k} {fmt} ticket {t}"
+    eng.close()
