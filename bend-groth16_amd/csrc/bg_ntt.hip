@@ -93,19 +93,64 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass(const uint32_t* src, u
     sh[e] = x;
   }
   __syncthreads();
-  const uint32_t nbf = cnt >> 1;
-  for (int s = 0; s < t; s++) {
+  // twiddle of stage k for local row residue jl: stage k's table holds omega^(j 2^k)
+  // contiguously from n - n/2^k, so the lanes of a row (consecutive c) read consecutive
+  // twiddles
+  auto twid = [&](int k, uint32_t jl, uint32_t c) {
+    return load_fp(tw + 8 * (((1ull << log2) - (1ull << (log2 - k))) + (((uint64_t)jl) << S_log) + q0 + c));
+  };
+  // Stages two at a time (radix 4): a thread takes the four rows j0, j0 + 2^ls, j0 + 2^(ls+1),
+  // j0 + 2^(ls+1) + 2^ls of one column, where 2^ls is the smaller of the two stages' local
+  // row distances, and applies both stages in registers: half the LDS round trips and
+  // barriers of one stage at a time, the same four twiddle products.
+  int s = 0;
+  for (; s + 1 < t; s += 2) {
+    const int ka = DIT ? k0 + t - 1 - s : k0 + s;  // applied first
+    const int kb = DIT ? ka - 1 : ka + 1;          // applied second
+    const int ls = log2 - 1 - (DIT ? ka : kb) - S_log;
+    const uint32_t ng = cnt >> 2;
+    for (uint32_t g = threadIdx.x; g < ng; g += NTT_THREADS) {
+      const uint32_t c = g & (C - 1), gj = g >> logC;
+      const uint32_t jl = gj & ((1u << ls) - 1);
+      const uint32_t j0 = ((gj >> ls) << (ls + 2)) + jl;
+      const uint32_t e0 = (j0 << logC) + c, ds = 1u << (ls + logC), db = ds << 1;
+      Fp x0 = sh[e0], x1 = sh[e0 + ds], x2 = sh[e0 + db], x3 = sh[e0 + db + ds];
+      if (DIT) {
+        // stage ka (distance ds): (x0, x1), (x2, x3); stage kb (distance db): (y0, y2), (y1, y3)
+        const Fp wa = twid(ka, jl, c);
+        Fp u = Fr::mul(x1, wa), v = Fr::mul(x3, wa);
+        const Fp y0 = Fr::add(x0, u), y1 = Fr::sub(x0, u), y2 = Fr::add(x2, v), y3 = Fr::sub(x2, v);
+        u = Fr::mul(y2, twid(kb, jl, c));
+        v = Fr::mul(y3, twid(kb, jl + (1u << ls), c));
+        sh[e0] = Fr::add(y0, u);
+        sh[e0 + db] = Fr::sub(y0, u);
+        sh[e0 + ds] = Fr::add(y1, v);
+        sh[e0 + db + ds] = Fr::sub(y1, v);
+      } else {
+        // stage ka (distance db): (x0, x2), (x1, x3); stage kb (distance ds): (y0, y1), (y2, y3)
+        const Fp y0 = Fr::add(x0, x2), y1 = Fr::add(x1, x3);
+        const Fp y2 = Fr::mul(Fr::sub(x0, x2), twid(ka, jl, c));
+        const Fp y3 = Fr::mul(Fr::sub(x1, x3), twid(ka, jl + (1u << ls), c));
+        const Fp wb = twid(kb, jl, c);
+        sh[e0] = Fr::add(y0, y1);
+        sh[e0 + ds] = Fr::mul(Fr::sub(y0, y1), wb);
+        sh[e0 + db] = Fr::add(y2, y3);
+        sh[e0 + db + ds] = Fr::mul(Fr::sub(y2, y3), wb);
+      }
+    }
+    __syncthreads();
+  }
+  if (s < t) {
+    // an odd stage count leaves one radix-2 stage
     const int k = DIT ? k0 + t - 1 - s : k0 + s;
     const int lmr = log2 - 1 - k - S_log;  // local row distance m / S = 2^lmr
+    const uint32_t nbf = cnt >> 1;
     for (uint32_t b = threadIdx.x; b < nbf; b += NTT_THREADS) {
       uint32_t c = b & (C - 1), bj = b >> logC;
       uint32_t jl = bj & ((1u << lmr) - 1);
       uint32_t j0 = ((bj >> lmr) << (lmr + 1)) + jl;
       uint32_t e0 = (j0 << logC) + c, e1 = e0 + (1u << (lmr + logC));
-      // stage k's table holds omega^(j 2^k) contiguously from n - n/2^k, so the lanes of a
-      // row (consecutive c) read consecutive twiddles
-      uint64_t ex = ((1ull << log2) - (1ull << (log2 - k))) + (((uint64_t)jl) << S_log) + q0 + c;
-      Fp w = load_fp(tw + 8 * ex);
+      Fp w = twid(k, jl, c);
       Fp u = sh[e0], v = sh[e1];
       if (DIT) {
         v = Fr::mul(v, w);
