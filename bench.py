@@ -6,7 +6,9 @@ SquareChain with N = 2^K constraints (x_{i+1} = x_i^2), domain d = 2^(K+1), prov
 from this library's seeded trapdoor setup (synthetic, seed 42), key resident in HBM and
 the packed witness already resident in HBM when the timed region starts.  A step is a
 full prove: evaluation vectors, qap_h (7 NTTs of 2^(K+1)), the five MSMs (A, B1, H, L in
-G1; B2 in G2) and the proof combination; the output is the affine proof (A, B, C).
+G1; B2 in G2) and the proof combination; the output is the affine proof (A, B, C).  Outside
+the timed region the proof is also made from the host witness (PCIe-inclusive time) and
+checked with the pairing equation against vk.bin (host verifier, "proof_verified").
 
 Prints one JSON line (rank 0).  With --gpus N > 1 it exits: this build measures one GPU.
 
@@ -583,6 +585,14 @@ def main() -> int:
     for k, v in sorted(alone.items()):
         log(f"  {k:22s} {v['avg_ms']:8.3f} ms/launch (n={v['n']})")
     roof = roofline(alone, peak)
+    # the benchmarked proof checked with the pairing equation (host verifier, vk.bin)
+    verified = None
+    if (key / "vk.bin").exists():
+        vk = np.fromfile(key / "vk.bin", dtype=np.uint32)
+        verified = bool(bendg.verify(vk, p_host, z16[16:32]))
+        log(f"proof verifies against vk.bin: {verified}")
+        if not verified:
+            return 1
     log("engine (concurrent proofs of this key)")
     batched = engine_rate(pk, dz, rs, 32, 4, ref)
     batched["vs_single"] = round(batched["proofs_per_s"] / (1e3 / ms), 3)
@@ -597,6 +607,7 @@ def main() -> int:
         "latency_ms": {"p50": round(float(np.percentile(walls, 50)), 3),
                        "p99": round(float(np.percentile(walls, 99)), 3)},
         "pcie_inclusive_ms": round(pcie_ms, 3),
+        "proof_verified": verified,
         "key_load_ms": round(load_ms, 1),
         "kernel_ms_per_proof_overlapped": per_proof,
         "kernel_ms_standalone": {k: v["avg_ms"] for k, v in alone.items()},
