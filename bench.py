@@ -403,6 +403,8 @@ def primitives_mode(a) -> dict:
     hip = Hip()
     L = bendg.lib()
     out = {}
+    peak = probe_peak() / 1e12  # T mul/s
+    out["mul_peak_T_per_s"] = round(peak, 3)
     rng = np.random.default_rng(a.seed)
     R = 21888242871839275222246405745257275088548364400416034343698204186575808495617
     flags = bendg.FMT_PACKED | bendg.MEM_DEVICE
@@ -439,7 +441,15 @@ def primitives_mode(a) -> dict:
         want = bendg.fixed_base_mul(group, kt).reshape(-1)
         ok = bool(np.array_equal(got, want))
         info = bendg.msm_plan_info(n, group)
-        out[f"msm_g{group}_2^{lg}"] = {"ms": round(ms, 3), "points_per_s": round(n / ms * 1e3, 0), "c": info["c"],
+        # algorithmic work at the planned c in Fq products (Pippenger with the reference's
+        # per-operation costs: G1 madd 11, add 16, dbl 7; G2 29 / 43 / 16), 136 multiplies each
+        c = info["c"]
+        w = -(-254 // c)
+        madd, add, dbl = (11, 16, 7) if group == 1 else (29, 43, 16)
+        fqm = n * w * madd + w * (1 << (c - 1)) * 2 * add + 254 * dbl + w * add
+        tmul = fqm * MULS_PER_FQ / (ms * 1e-3) / 1e12
+        out[f"msm_g{group}_2^{lg}"] = {"ms": round(ms, 3), "points_per_s": round(n / ms * 1e3, 0), "c": c,
+                                       "algorithmic_T_mul_per_s": round(tmul, 2), "valu_frac": round(tmul / peak, 3),
                                        "closed_form_ok": ok}
         log(f"  MSM G{group} 2^{lg}: {ms:.3f} ms, closed form {'ok' if ok else 'MISMATCH'}")
         for p in (dp, ds, dout):
@@ -466,8 +476,15 @@ def primitives_mode(a) -> dict:
         ok = bool(np.array_equal(back, x))
         hip.free(d)
         byts = 64 * n
+        # both roofs for the transform: algorithmic HBM bytes (one 32 B read and write per
+        # element) against 8 TB/s, and (n/2) log2 n Fr products x 136 multiplies against the
+        # measured multiply peak
+        gbps = byts / res["fft"] / 1e6
+        tmul = (n // 2) * lg * MULS_PER_FQ / (res["fft"] * 1e-3) / 1e12
         out[f"ntt_2^{lg}"] = {"fft_ms": round(res["fft"], 3), "ifft_ms": round(res["ifft"], 3),
-                              "fft_GBps_algorithmic": round(byts / res["fft"] / 1e6, 1), "round_trip_ok": ok}
+                              "fft_GBps_algorithmic": round(gbps, 1), "hbm_frac": round(gbps / 8000.0, 4),
+                              "fft_T_mul_per_s": round(tmul, 2), "valu_frac": round(tmul / peak, 3),
+                              "round_trip_ok": ok}
         log(f"  NTT 2^{lg}: fft {res['fft']:.3f} ms, ifft {res['ifft']:.3f} ms, round trip {'ok' if ok else 'BAD'}")
         if not ok:
             raise SystemExit("NTT round trip failed")
