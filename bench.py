@@ -92,8 +92,9 @@ def ensure_key(k: int, seed: int, base: Path) -> Path:
     return d
 
 
-# field products per group operation, and 32x32->64 multiplies per Fq product (64 a*b,
-# 64 m*p, 8 reduction digits)
+# field products per group operation, and the algorithmic 32x32->64 multiplies per Fq
+# product: 64 a*b, 64 m*p and 8 reduction digits of the 32-bit-digit Montgomery product (the
+# work unit; the kernels' 29-bit-digit product issues 171 multiplies for the same product)
 MULS_PER_FQ = 136
 FQ_PER_MADD = {1: 10, 2: 28}   # madd-2008-s: 8M + 2S (G2: Karatsuba 3 / complex square 2)
 
