@@ -41,9 +41,11 @@ static void finish_plan(MsmPlan& p, int group) {
   p.nb = p.M * p.kspan;
   // buckets per level-1 group: 2, 4 or 8 (the four-role level-1 sums need m <= 8)
   // G1's level 1 runs one wave per binary-weight role, so wide groups cost no depth; G2's
-  // runs one lane per group (a running sum of depth 2m), so its groups are narrower
+  // runs one lane per group (a running sum of depth 2m), so with one set of buckets
+  // (fixed-base tables) its groups are narrower; with a bucket set per window the level-2
+  // blocks of narrow groups would outnumber what the device holds at once
   uint32_t mm = (uint32_t)tune_env(group == 1 ? "BG16_MSM_M" : "BG16_MSM_M_G2");
-  if (mm == 0) mm = group == 1 ? 8 : 2;
+  if (mm == 0) mm = group == 1 || p.W > 1 ? 8 : 2;
   mm = mm >= 8 ? 8 : (mm >= 4 ? 4 : 2);
   p.m = std::min<uint32_t>(mm, p.B);
   p.G = p.B / p.m;
