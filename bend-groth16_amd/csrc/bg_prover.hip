@@ -390,7 +390,8 @@ void prove_on(bg_pk* pk, ProveSlot& sl, const uint32_t* witness, const uint32_t*
     hipEvent_t* ev = sl.ev;
     sl.ensure(nz, d);
     BG_HIP(hipEventRecord(ev[0], s0));
-    // 1. witness upload (16-bit words are packed on the device)
+    // 1. witness upload (a host witness is packed by the worker pool into pinned staging
+    //    and sent in one copy; a device witness is copied in place)
     uint32_t* z = sl.z.as<uint32_t>();
     upload_witness(witness, flags, nz, z, static_cast<uint32_t*>(sl.zpin.get(32 * nz)), s0);
     BG_HIP(hipEventRecord(ev[1], s0));
