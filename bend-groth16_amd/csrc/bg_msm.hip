@@ -43,9 +43,12 @@ static void finish_plan(MsmPlan& p, int group) {
   // G1's level 1 runs one wave per binary-weight role, so wide groups cost no depth; G2's
   // runs one lane per group (a running sum of depth 2m), so with one set of buckets
   // (fixed-base tables) its groups are narrower; with a bucket set per window the level-2
-  // blocks of narrow groups would outnumber what the device holds at once
+  // blocks of narrow groups would outnumber what the device holds at once.  A fixed-base
+  // set of 2^16 or more buckets takes groups of 4: in pairs its level 2 would hold 512
+  // waves of G2 registers (a SIMD's file each) for ~0.65 ms while the other MSMs of a
+  // proof accumulate (2^18 proof 5.22 -> 5.12 ms; below 2^16 buckets pairs stay faster)
   uint32_t mm = (uint32_t)tune_env(group == 1 ? "BG16_MSM_M" : "BG16_MSM_M_G2");
-  if (mm == 0) mm = group == 1 || p.W > 1 ? 8 : 2;
+  if (mm == 0) mm = group == 1 || p.W > 1 ? 8 : (p.B >= 65536 ? 4 : 2);
   mm = mm >= 8 ? 8 : (mm >= 4 ? 4 : 2);
   p.m = std::min<uint32_t>(mm, p.B);
   p.G = p.B / p.m;

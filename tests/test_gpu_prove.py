@@ -127,6 +127,21 @@ def test_prove_windowed_and_fixed_base_agree(bg, keys, monkeypatch):
     pk_plain.close()
 
 
+@pytest.mark.parametrize("m", ["2", "4", "8"])
+def test_prove_g2_group_widths_agree(bg, keys, monkeypatch, m):
+    """The G2 bucket reduction's level-1 group width (pairs below 2^16 fixed-base buckets,
+    fours above, eights with a bucket set per window) does not change the proof."""
+    d = keys[12]
+    z, rs = _files(d)
+    pk = bendg.ProvingKey(d)
+    ref, _ = pk.prove(z, rs)
+    monkeypatch.setenv("BG16_MSM_M_G2", m)
+    got, _ = pk.prove(z, rs)
+    monkeypatch.delenv("BG16_MSM_M_G2")
+    assert np.array_equal(got, ref)
+    pk.close()
+
+
 def test_cli_prove_and_verify(bg, keys):
     """bg16_prove (the bend2/prove.bend command-line shape) prints the library's proof and,
     with --verify, checks it against vk.bin."""
