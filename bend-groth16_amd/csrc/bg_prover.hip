@@ -515,24 +515,33 @@ void prove_on(bg_pk* pk, ProveSlot& sl, const uint32_t* witness, const uint32_t*
     bgh::P2 s_d2 = delta2.mul(sc);
 
     // 5. finish the MSMs and combine (groth16.bend:298-308); slots as prove.bend:64-116.
-    //    Everything but the H term is combined while MSM H still runs on the device.
-    uint32_t g1res[4 * 16], mB2[32], mHs[16];
-    msm_finish(sl.work[0], g1res);
-    const uint32_t *mA = g1res, *mB1 = g1res + 16, *mL = g1res + 32;
-    const uint32_t* mH = g1res + 48;
-    bgh::P1 A = bgh::load_affine<bgh::Fq>(mA).add(r_d1).add(alpha);
-    bgh::P1 B1 = bgh::load_affine<bgh::Fq>(mB1).add(s_d1).add(beta1);
-    bgh::P1 C0 = A.mul(sc).add(B1.mul(rc)).add(rs_d1.neg()).add(bgh::load_affine<bgh::Fq>(mL));
+    //    Everything but the H term is combined while MSM H still runs on the device: B2
+    //    first (its stream finishes first), then the G1 batch, whose s A and r B1 run on
+    //    two workers.  MSM results stay in XYZZ form; only A, B2 and C are made affine.
+    uint32_t g1x[4 * 32], b2x[64], hx[32];
     uint32_t proof[64];
-    bgh::store_affine(A, proof);
-    msm_finish(sl.work[2], mB2);
-    bgh::P2 B2 = bgh::load_affine<bgh::Fq2>(mB2).add(s_d2).add(beta2);
+    msm_finish(sl.work[2], nullptr, b2x);
+    bgh::P2 B2 = bgh::load_xyzz<bgh::Fq2>(b2x).add(s_d2).add(beta2);
     bgh::store_affine(B2, proof + 16);
+    msm_finish(sl.work[0], nullptr, g1x);
+    const uint32_t *xA = g1x, *xB1 = g1x + 32, *xL = g1x + 64;
+    const uint32_t* xH = g1x + 96;
+    bgh::P1 A = bgh::load_xyzz<bgh::Fq>(xA).add(r_d1).add(alpha);
+    bgh::P1 B1 = bgh::load_xyzz<bgh::Fq>(xB1).add(s_d1).add(beta1);
+    bgh::P1 sA, rB1;
+    host_parallel(2, [&](int i) {
+      if (i == 0)
+        sA = A.mul(sc);
+      else
+        rB1 = B1.mul(rc);
+    });
+    bgh::P1 C0 = sA.add(rB1).add(rs_d1.neg()).add(bgh::load_xyzz<bgh::Fq>(xL));
+    bgh::store_affine(A, proof);
     if (sched != 1 || pk->fixed) {
-      msm_finish(sl.work[1], mHs);
-      mH = mHs;
+      msm_finish(sl.work[1], nullptr, hx);
+      xH = hx;
     }
-    bgh::P1 C = C0.add(bgh::load_affine<bgh::Fq>(mH));
+    bgh::P1 C = C0.add(bgh::load_xyzz<bgh::Fq>(xH));
     bgh::store_affine(C, proof + 48);
     if (words16) {
       for (int i = 0; i < 64; i++) {
