@@ -52,7 +52,13 @@ static void finish_plan(MsmPlan& p, int group) {
   mm = mm >= 8 ? 8 : (mm >= 4 ? 4 : 2);
   p.m = std::min<uint32_t>(mm, p.B);
   p.G = p.B / p.m;
+  // level-2 block: as wide as the LDS holds, except for a G2 fixed-base set of at most
+  // 1024 groups (the B2 of proofs up to 2^11), where the level 2 is a few blocks whose
+  // chain of 2 log2 K G2 additions is the latency: blocks of 32 shorten it and leave the
+  // host at most 32 outputs (2^10 proof 1.36 -> 1.21 ms; at 2048 groups the extra host
+  // work cancels it)
   uint32_t kmax = group == 1 ? 256 : 128;
+  if (group == 2 && p.W == 1 && p.G <= 1024) kmax = 32;
   uint32_t kk = (uint32_t)tune_env("BG16_MSM_K");
   if (kk) kmax = std::min(kmax, kk);
   p.K = std::min<uint32_t>(kmax, p.G);
