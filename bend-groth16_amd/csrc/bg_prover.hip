@@ -119,11 +119,13 @@ __global__ void k_from_mont(const uint32_t* __restrict__ in, uint32_t* __restric
 
 static unsigned nb(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
-// Order of qap_h and the MSMs inside one proof (see bg_prove); BG16_PROVE_SCHEDULE=0..4.
+// Order of qap_h and the MSMs inside one proof (see prove_on); BG16_PROVE_SCHEDULE=0..5,
+// any other value keeps the default 0.
 static int prove_schedule() {
   static int s = [] {
     const char* e = getenv("BG16_PROVE_SCHEDULE");
-    return e ? atoi(e) : 0;
+    const int v = e ? atoi(e) : 0;
+    return v >= 0 && v <= 5 ? v : 0;
   }();
   return s;
 }

@@ -160,3 +160,20 @@ def test_cli_prove_and_verify(bg, keys):
     assert lines[:3] == want
     assert lines[-1] == "V ok"
     assert any(l.startswith("T total ") for l in lines)
+
+
+@pytest.mark.parametrize("sched", ["5", "9"])
+def test_cli_schedules(bg, keys, sched):
+    """Every proof schedule gives the same proof (5: one stream); a value outside 0..5
+    keeps the default."""
+    import os
+    import subprocess
+    from pathlib import Path
+    cli = Path(bendg.__file__).resolve().parent.parent / "bg16_prove"
+    d = keys[10]
+    ref = subprocess.run([str(cli), str(d)], capture_output=True, text=True, timeout=120)
+    env = dict(os.environ, BG16_PROVE_SCHEDULE=sched)
+    out = subprocess.run([str(cli), "--verify", str(d)], capture_output=True, text=True, timeout=120, env=env)
+    assert ref.returncode == 0 and out.returncode == 0, out.stderr
+    assert out.stdout.split("\n")[:3] == ref.stdout.split("\n")[:3]
+    assert out.stdout.strip().split("\n")[-1] == "V ok"
