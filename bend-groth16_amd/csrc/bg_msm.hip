@@ -59,8 +59,14 @@ static void finish_plan(MsmPlan& p, int group) {
   // work cancels it)
   uint32_t kmax = group == 1 ? 256 : 128;
   if (group == 2 && p.W == 1 && p.G <= 1024) kmax = 32;
-  uint32_t kk = (uint32_t)tune_env("BG16_MSM_K");
-  if (kk) kmax = std::min(kmax, kk);
+  // override: rounded down to a power of two (the level-2 scan and trees need one, and
+  // K must divide the group count)
+  uint32_t kk = (uint32_t)std::max(0, tune_env("BG16_MSM_K"));
+  if (kk) {
+    uint32_t k2 = 1;
+    while (k2 * 2 <= kk) k2 *= 2;
+    kmax = std::min(kmax, k2);
+  }
   p.K = std::min<uint32_t>(kmax, p.G);
   p.H = p.G / p.K;
   p.entries = (uint64_t)p.Wd * p.n;
@@ -131,7 +137,7 @@ MsmPlan msm_plan_batch(const uint64_t* ns, uint32_t M, int group, int window_bit
 
 int fixed_window_bits(const uint64_t* ns, uint32_t M, int group) {
   int env = tune_env(group == 1 ? "BG16_FIXED_C_G1" : "BG16_FIXED_C_G2");
-  if (env > 0) return env;
+  if (env >= 2 && env <= 22) return env;  // other values: the planner's width
   // every window shares one set of buckets
   double best = 1e300;
   int c = 16;

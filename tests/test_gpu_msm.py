@@ -219,3 +219,18 @@ def test_msm_short_top_window(bg, group, log2n, c):
     sc = O.rand_scalars(rng, n)
     print(f"msm top window G{group} 2^{log2n} c={c}", flush=True)
     assert _closed(group, pts, ks, sc, window_bits=c)
+
+
+@pytest.mark.parametrize("group", [1, 2], ids=["G1", "G2"])
+@pytest.mark.parametrize("kval", ["1", "3", "100", "4096"])
+def test_msm_reduction_block_override(bg, monkeypatch, group, kval):
+    """BG16_MSM_K (level-2 block width) is rounded down to a power of two and capped by
+    the group count, so any value keeps the result exact."""
+    rng = np.random.default_rng(300 + group)
+    n = 3000
+    pts, _ = _bases(group, rng, n)
+    sc = O.rand_scalars(rng, n)
+    monkeypatch.setenv("BG16_MSM_K", kval)
+    got = _msm(group, pts, sc)
+    monkeypatch.delenv("BG16_MSM_K")
+    assert np.array_equal(got, _oracle(group, pts, sc))
