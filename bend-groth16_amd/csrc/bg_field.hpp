@@ -285,17 +285,22 @@ struct Fq2 {
   __device__ __forceinline__ static Fp2 sub(const Fp2& a, const Fp2& b) { return {Fq::sub(a.c0, b.c0), Fq::sub(a.c1, b.c1)}; }
   __device__ __forceinline__ static Fp2 neg(const Fp2& a) { return {Fq::neg(a.c0), Fq::neg(a.c1)}; }
   __device__ __forceinline__ static Fp2 dbl(const Fp2& a) { return {Fq::dbl(a.c0), Fq::dbl(a.c1)}; }
-  // Karatsuba: 3 Fq products.  The scheduling barriers keep the three products from
-  // being interleaved, which would hold three accumulators and operand sets live at once
-  // (the G2 accumulation then no longer fits two waves per SIMD).
+  // Two sums of two products, one Montgomery reduction each (bg_mul29.hpp mont2):
+  //   c0 = a0 b0 + a1 (2p - b1),  c1 = a0 b1 + a1 b0,
+  // each below 2.6 p and brought below 2p by one conditional subtraction: the same 4 x 81
+  // digit products as Karatsuba's 3 x 81 plus its 81 of the third reduction, but two
+  // reductions instead of three and no Karatsuba sums and differences.  The scheduling
+  // barrier keeps the two halves from being interleaved (two operand sets live at once).
   __device__ __forceinline__ static Fp2 mul(const Fp2& a, const Fp2& b) {
-    Fp t0 = Fq::mul(a.c0, b.c0);
+    const Fp nb1 = Fq::neg(b.c1);
+    Fp t, c0, c1;
+    m29::mul2_words(a.c0.v, b.c0.v, a.c1.v, nb1.v, t.v, FqP::p29, FqP::n0_29);
+    reduce_2p<FqP>(c0.v, t.v);
     __builtin_amdgcn_sched_barrier(0);
-    Fp t1 = Fq::mul(a.c1, b.c1);
+    m29::mul2_words(a.c0.v, b.c1.v, a.c1.v, b.c0.v, t.v, FqP::p29, FqP::n0_29);
+    reduce_2p<FqP>(c1.v, t.v);
     __builtin_amdgcn_sched_barrier(0);
-    Fp t2 = Fq::mul(Fq::add(a.c0, a.c1), Fq::add(b.c0, b.c1));
-    __builtin_amdgcn_sched_barrier(0);
-    return {Fq::sub(t0, t1), Fq::sub(Fq::sub(t2, t0), t1)};
+    return {c0, c1};
   }
   // complex squaring: 2 Fq products
   __device__ __forceinline__ static Fp2 sqr(const Fp2& a) {

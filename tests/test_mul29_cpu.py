@@ -24,6 +24,10 @@ SRC = """
 extern "C" void mulw(const uint32_t* a, const uint32_t* b, uint32_t* o, const uint32_t* p29, uint32_t n0) {
   bg::m29::mul_words(a, b, o, p29, n0);
 }
+extern "C" void mul2w(const uint32_t* a1, const uint32_t* b1, const uint32_t* a2, const uint32_t* b2, uint32_t* o,
+                      const uint32_t* p29, uint32_t n0) {
+  bg::m29::mul2_words(a1, b1, a2, b2, o, p29, n0);
+}
 """
 
 
@@ -69,3 +73,25 @@ def test_montgomery_product_29bit_digits(lib, p):
         r = value(out)
         assert r % p == a * b * rinv % p, (hex(a), hex(b))
         assert r < 2 * p, (hex(a), hex(b), r / p)
+
+
+@pytest.mark.parametrize("p", [P_R, P_Q], ids=["Fr", "Fq"])
+def test_sum_of_two_products(lib, p):
+    """(a1 b1 + a2 b2) 2^-256 mod p with one reduction, below 2.6 p for operands below 2p
+    (the Fq2 product's two components), at the extremes and at random."""
+    n0 = (-pow(p, -1, 1 << 29)) % (1 << 29)
+    p29 = digits29(p)
+    rinv = pow(1 << 256, -1, p)
+    rng = random.Random(p & 0xFFF)
+    edge = [0, 1, p - 1, p, 2 * p - 1, (1 << 253)]
+    cases = [(a, b, c, d) for a in edge for b in edge for c in (0, 2 * p - 1) for d in (1, 2 * p - 1)]
+    cases += [tuple(rng.randrange(2 * p) for _ in range(4)) for _ in range(3000)]
+    out = W8()
+    worst = 0.0
+    for a1, b1, a2, b2 in cases:
+        lib.mul2w(words(a1), words(b1), words(a2), words(b2), out, p29, n0)
+        r = value(out)
+        assert r % p == (a1 * b1 + a2 * b2) * rinv % p
+        assert r < 2.6 * p
+        worst = max(worst, r / p)
+    assert worst > 1.0  # the bound is exercised: results above p occur
