@@ -54,7 +54,7 @@ struct Curve {
     E x2 = F::sqr(a.x);
     E m = F::add(F::dbl(x2), x2);
     E x3 = F::sub(F::sqr(m), F::dbl(s));
-    E y3 = F::sub(F::mul(m, F::sub(s, x3)), F::mul(w, a.y));
+    E y3 = F::mul_sub(m, F::sub(s, x3), w, a.y);
     return {x3, y3, v, w};
   }
 
@@ -67,7 +67,7 @@ struct Curve {
     E x2 = F::sqr(p.X);
     E m = F::add(F::dbl(x2), x2);
     E x3 = F::sub(F::sqr(m), F::dbl(s));
-    E y3 = F::sub(F::mul(m, F::sub(s, x3)), F::mul(w, p.Y));
+    E y3 = F::mul_sub(m, F::sub(s, x3), w, p.Y);
     return {x3, y3, F::mul(v, p.ZZ), F::mul(w, p.ZZZ)};
   }
 
@@ -87,7 +87,7 @@ struct Curve {
     E ppp = F::mul(pp_, pp);
     E q = F::mul(p.X, pp);
     E x3 = F::sub(F::sub(F::sqr(r), ppp), F::dbl(q));
-    E y3 = F::sub(F::mul(r, F::sub(q, x3)), F::mul(p.Y, ppp));
+    E y3 = F::mul_sub(r, F::sub(q, x3), p.Y, ppp);
     return {x3, y3, F::mul(p.ZZ, pp), F::mul(p.ZZZ, ppp)};
   }
 
@@ -106,7 +106,7 @@ struct Curve {
     E zzz3 = F::mul(p.ZZZ, ppp);
     E q = F::mul(p.X, pp);
     E x3 = F::sub(F::sub(F::sqr(r), ppp), F::dbl(q));
-    E y3 = F::sub(F::mul(r, F::sub(q, x3)), F::mul(p.Y, ppp));
+    E y3 = F::mul_sub(r, F::sub(q, x3), p.Y, ppp);
     return {x3, y3, zz3, zzz3};
   }
 
@@ -126,7 +126,7 @@ struct Curve {
     E zz3 = F::mul(F::mul(p.ZZ, q.ZZ), pp);
     E zzz3 = F::mul(F::mul(p.ZZZ, q.ZZZ), ppp);
     E x3 = F::sub(F::sub(F::sqr(r), ppp), F::dbl(q_));
-    E y3 = F::sub(F::mul(r, F::sub(q_, x3)), F::mul(s1, ppp));
+    E y3 = F::mul_sub(r, F::sub(q_, x3), s1, ppp);
     return {x3, y3, zz3, zzz3};
   }
 
@@ -157,7 +157,7 @@ struct Curve {
     E ppp = F::mul(pp_, pp);
     E q_ = F::mul(u1, pp);
     E x3 = F::sub(F::sub(F::sqr(r), ppp), F::dbl(q_));
-    E y3 = F::sub(F::mul(r, F::sub(q_, x3)), F::mul(s1, ppp));
+    E y3 = F::mul_sub(r, F::sub(q_, x3), s1, ppp);
     return {x3, y3, F::mul(F::mul(p.ZZ, q.ZZ), pp), F::mul(F::mul(p.ZZZ, q.ZZZ), ppp)};
   }
 

@@ -212,6 +212,16 @@ struct Field {
 
   __device__ __forceinline__ static Fp sqr(const Fp& a) { return mul(a, a); }
 
+  // a b - c d as a b + c (2p - d) with one reduction for both products (below 2.6 p, then
+  // one conditional subtraction): the y coordinate of every XYZZ addition and doubling
+  __device__ __forceinline__ static Fp mul_sub(const Fp& a, const Fp& b, const Fp& c, const Fp& d) {
+    const Fp nd = neg(d);
+    Fp t, r;
+    m29::mul2_words(a.v, b.v, c.v, nd.v, t.v, P::p29, P::n0_29);
+    reduce_2p<P>(r.v, t.v);
+    return r;
+  }
+
   // x^e for an exponent given as 8 little-endian words (left to right)
   __device__ __noinline__ static Fp pow(const Fp& x, const uint32_t* e) {
     Fp acc = one();
@@ -301,6 +311,11 @@ struct Fq2 {
     reduce_2p<FqP>(c1.v, t.v);
     __builtin_amdgcn_sched_barrier(0);
     return {c0, c1};
+  }
+  // a b - c d (four Fq products summed per component would exceed one conditional
+  // subtraction's range, so the two Fq2 products stay separate)
+  __device__ __forceinline__ static Fp2 mul_sub(const Fp2& a, const Fp2& b, const Fp2& c, const Fp2& d) {
+    return sub(mul(a, b), mul(c, d));
   }
   // complex squaring: 2 Fq products
   __device__ __forceinline__ static Fp2 sqr(const Fp2& a) {
