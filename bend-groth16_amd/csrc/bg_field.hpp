@@ -312,10 +312,30 @@ struct Fq2 {
     __builtin_amdgcn_sched_barrier(0);
     return {c0, c1};
   }
-  // a b - c d (four Fq products summed per component would exceed one conditional
-  // subtraction's range, so the two Fq2 products stay separate)
+  // a b - c d with one reduction per component (bg_mul29.hpp mont4):
+  //   r0 = a0 b0 + a1 (2p - b1) + c0 (2p - d0) + c1 d1
+  //   r1 = a0 b1 + a1 b0 + c0 (2p - d1) + c1 (2p - d0)
+  // each below 4.1 p, brought below 2p by two conditional subtractions
   __device__ __forceinline__ static Fp2 mul_sub(const Fp2& a, const Fp2& b, const Fp2& c, const Fp2& d) {
-    return sub(mul(a, b), mul(c, d));
+    const Fp nb1 = Fq::neg(b.c1), nd0 = Fq::neg(d.c0), nd1 = Fq::neg(d.c1);
+    Fp t, r0, r1;
+    {
+      const uint32_t* x[4] = {a.c0.v, a.c1.v, c.c0.v, c.c1.v};
+      const uint32_t* y[4] = {b.c0.v, nb1.v, nd0.v, d.c1.v};
+      m29::mul4_words(x, y, t.v, FqP::p29, FqP::n0_29);
+      reduce_2p<FqP>(t.v, t.v);
+      reduce_2p<FqP>(r0.v, t.v);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    {
+      const uint32_t* x[4] = {a.c0.v, a.c1.v, c.c0.v, c.c1.v};
+      const uint32_t* y[4] = {b.c1.v, b.c0.v, nd1.v, nd0.v};
+      m29::mul4_words(x, y, t.v, FqP::p29, FqP::n0_29);
+      reduce_2p<FqP>(t.v, t.v);
+      reduce_2p<FqP>(r1.v, t.v);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    return {r0, r1};
   }
   // complex squaring: 2 Fq products
   __device__ __forceinline__ static Fp2 sqr(const Fp2& a) {

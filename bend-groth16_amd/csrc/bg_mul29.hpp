@@ -134,6 +134,43 @@ BG_HD Digits mont2(const Digits& x1, const Digits& y1, const Digits& x2, const D
   return r;
 }
 
+// (x1 y1 + x2 y2 + x3 y3 + x4 y4) * 2^-261 mod p, one reduction for four products (the
+// G2 y coordinate).  A column holds up to 36 digit products and 9 reduction products,
+// each below 2^58: below 2^63.5 with the carry.  For x_k = 32 a_k, a_k, y_k < 2p the
+// result is below 512 p^2 / 2^261 + p < 4.1 p.
+BG_HD Digits mont4(const Digits* x, const Digits* y, const uint32_t* p29, uint32_t n0) {
+  uint32_t m[9];
+  Digits r;
+  uint64_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+#pragma unroll
+    for (int i = 0; i < k; i++) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) acc += (uint64_t)x[q].d[i] * y[q].d[k - i];
+      acc += (uint64_t)m[i] * p29[k - i];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc += (uint64_t)x[q].d[k] * y[q].d[0];
+    m[k] = ((uint32_t)acc * n0) & MASK;
+    acc += (uint64_t)m[k] * p29[0];
+    acc >>= 29;
+  }
+#pragma unroll
+  for (int k = 9; k < 17; k++) {
+#pragma unroll
+    for (int i = k - 8; i < 9; i++) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) acc += (uint64_t)x[q].d[i] * y[q].d[k - i];
+      acc += (uint64_t)m[i] * p29[k - i];
+    }
+    r.d[k - 9] = (uint32_t)acc & MASK;
+    acc >>= 29;
+  }
+  r.d[8] = (uint32_t)acc;
+  return r;
+}
+
 // a * b * 2^-256 mod p on 8-word operands
 BG_HD void mul_words(const uint32_t* a, const uint32_t* b, uint32_t* out, const uint32_t* p29, uint32_t n0) {
   pack(mont(split<5>(a), split<0>(b), p29, n0), out);
@@ -143,6 +180,18 @@ BG_HD void mul_words(const uint32_t* a, const uint32_t* b, uint32_t* out, const 
 BG_HD void mul2_words(const uint32_t* a1, const uint32_t* b1, const uint32_t* a2, const uint32_t* b2, uint32_t* out,
                       const uint32_t* p29, uint32_t n0) {
   pack(mont2(split<5>(a1), split<0>(b1), split<5>(a2), split<0>(b2), p29, n0), out);
+}
+
+// (sum of a_k * b_k, k < 4) * 2^-256 mod p on 8-word operands below 2p, result below 4.1 p
+BG_HD void mul4_words(const uint32_t* const* a, const uint32_t* const* b, uint32_t* out, const uint32_t* p29,
+                      uint32_t n0) {
+  Digits x[4], y[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    x[q] = split<5>(a[q]);
+    y[q] = split<0>(b[q]);
+  }
+  pack(mont4(x, y, p29, n0), out);
 }
 
 }  // namespace m29

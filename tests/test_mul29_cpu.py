@@ -24,6 +24,11 @@ SRC = """
 extern "C" void mulw(const uint32_t* a, const uint32_t* b, uint32_t* o, const uint32_t* p29, uint32_t n0) {
   bg::m29::mul_words(a, b, o, p29, n0);
 }
+extern "C" void mul4w(const uint32_t* a, const uint32_t* b, uint32_t* o, const uint32_t* p29, uint32_t n0) {
+  const uint32_t* as[4] = {a, a + 8, a + 16, a + 24};
+  const uint32_t* bs[4] = {b, b + 8, b + 16, b + 24};
+  bg::m29::mul4_words(as, bs, o, p29, n0);
+}
 extern "C" void mul2w(const uint32_t* a1, const uint32_t* b1, const uint32_t* a2, const uint32_t* b2, uint32_t* o,
                       const uint32_t* p29, uint32_t n0) {
   bg::m29::mul2_words(a1, b1, a2, b2, o, p29, n0);
@@ -95,3 +100,29 @@ def test_sum_of_two_products(lib, p):
         assert r < 2.6 * p
         worst = max(worst, r / p)
     assert worst > 1.0  # the bound is exercised: results above p occur
+
+
+@pytest.mark.parametrize("p", [P_R, P_Q], ids=["Fr", "Fq"])
+def test_sum_of_four_products(lib, p):
+    """(sum of four a_k b_k) 2^-256 mod p with one reduction, below 4.1 p for operands
+    below 2p (the G2 y coordinate's components), at the extremes and at random."""
+    n0 = (-pow(p, -1, 1 << 29)) % (1 << 29)
+    p29 = digits29(p)
+    rinv = pow(1 << 256, -1, p)
+    rng = random.Random(p & 0xFFFF)
+    W32 = ctypes.c_uint32 * 32
+    top = 2 * p - 1
+    cases = [[top] * 8, [0] * 8, [1] * 8, [p] * 8]
+    cases += [[rng.randrange(2 * p) for _ in range(8)] for _ in range(2000)]
+    cases += [[rng.choice([0, 1, p - 1, p, top]) for _ in range(8)] for _ in range(300)]
+    out = W8()
+    worst = 0.0
+    for v in cases:
+        a = W32(*[w for x in v[:4] for w in words(x)])
+        b = W32(*[w for x in v[4:] for w in words(x)])
+        lib.mul4w(a, b, out, p29, n0)
+        r = value(out)
+        assert r % p == sum(x * y for x, y in zip(v[:4], v[4:])) * rinv % p
+        assert r < 4.1 * p
+        worst = max(worst, r / p)
+    assert worst > 2.0  # results above 2p occur: two conditional subtractions are needed
