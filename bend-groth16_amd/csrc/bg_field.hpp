@@ -297,10 +297,11 @@ struct Fq2 {
   __device__ __forceinline__ static Fp2 dbl(const Fp2& a) { return {Fq::dbl(a.c0), Fq::dbl(a.c1)}; }
   // Two sums of two products, one Montgomery reduction each (bg_mul29.hpp mont2):
   //   c0 = a0 b0 + a1 (2p - b1),  c1 = a0 b1 + a1 b0,
-  // each below 2.6 p and brought below 2p by one conditional subtraction: the same 4 x 81
-  // digit products as Karatsuba's 3 x 81 plus its 81 of the third reduction, but two
-  // reductions instead of three and no Karatsuba sums and differences.  The scheduling
-  // barrier keeps the two halves from being interleaved (two operand sets live at once).
+  // each below 2.6 p and brought below 2p by one conditional subtraction.  4 x 81 digit
+  // products and 2 x 81 reduction products: as many multiplies as Karatsuba's 3 x (81 + 81),
+  // but two reductions instead of three and no Karatsuba sums and differences.  The
+  // scheduling barrier keeps the two halves from being interleaved (two operand sets live
+  // at once).
   __device__ __forceinline__ static Fp2 mul(const Fp2& a, const Fp2& b) {
     const Fp nb1 = Fq::neg(b.c1);
     Fp t, c0, c1;
