@@ -62,15 +62,17 @@ struct bg_engine {
 
 namespace bg {
 
-// Proofs of one shape a slot takes together: 16 up to domain 2^14, 8 up to 2^17, 4 at 2^18,
-// and none from 2^19, where one proof fills the device (a batch also completes all at once,
-// so large proofs stay alone).  BG16_ENGINE_BATCH caps it (1 disables batching).
+// Proofs of one shape a slot takes together: 16 up to domain 2^14, 4 up to 2^16, 2 up to
+// 2^18, and none from 2^19, where one proof fills the device.  Smaller batches as proofs
+// grow keep several slots' batches in flight (a batch completes all at once): bursts of 32
+// same-key proofs on 4 slots measured best at these sizes (2^15: 4 -> 1,175 proofs/s,
+// 16 -> 798; 2^17: 2 -> 685, 8 -> 405; 2^19: 1 -> 212, 2 -> 200).  BG16_ENGINE_BATCH caps
+// it (1 disables batching).
 static size_t batch_limit(bg_pk* pk) {
   uint32_t hdr[8];
   if (bg_pk_header(pk, hdr) != 0) return 1;
   const uint32_t lg = hdr[3];  // log2 of the domain: constraints + 2 rounded up
-  // (SquareChain 2^18, domain 2^19: pairs measured 200 proofs/s, single proofs 212)
-  size_t lim = lg <= 14 ? 16 : lg <= 17 ? 8 : lg == 18 ? 4 : 1;
+  size_t lim = lg <= 14 ? 16 : lg <= 16 ? 4 : lg <= 18 ? 2 : 1;
   const char* env = getenv("BG16_ENGINE_BATCH");
   if (env && atoi(env) > 0) lim = std::min<size_t>(lim, (size_t)atoi(env));
   return lim;
