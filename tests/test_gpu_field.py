@@ -128,3 +128,29 @@ def test_probe_rates(bg):
     print(f"\nv_mad_u64_u32: {mad / 1e12:.2f} T/s ({ms0:.2f} ms); Fq mul: {fq / 1e9:.1f} G/s ({ms1:.2f} ms); "
           f"G1 madd: {madd / 1e9:.2f} G/s ({ms2:.2f} ms)", flush=True)
     assert mad > 0 and fq > 0 and madd > 0
+
+
+def test_fq2_mul_lazy_range(bg):
+    """Fq2 products (two sums of two products, one reduction each) with operands anywhere in
+    the lazy range [0, 2p) of the Montgomery words -- w and w + p for the same value -- equal
+    the oracle's, so the reduction's bound (below 2.6 p before the conditional subtraction)
+    holds at its extremes."""
+    rnd = random.Random(17)
+    p = O.P_Q
+    n = 256
+    a = [(rnd.randrange(p), rnd.randrange(p)) for _ in range(n)]
+    b = [(rnd.randrange(p), rnd.randrange(p)) for _ in range(n)]
+    a[:4] = [(p - 1, p - 1), (0, p - 1), (p - 1, 0), (1, 1)]
+    b[:4] = [(p - 1, p - 1), (p - 1, p - 1), (p - 1, 1), (p - 1, p - 1)]
+
+    def enc_lazy(vals, k):
+        words = ints_u32([c for x in vals for c in x], p)  # Montgomery form, canonical
+        ints = u32_ints(words)
+        lifted = [w + p if (i + k) % 3 else w for i, w in enumerate(ints)]  # w + p < 2p
+        return ints_u32(lifted).reshape(-1, 16)
+
+    dec = lambda arr: [tuple(x) for x in np.array(u32_ints(arr, p), dtype=object).reshape(-1, 2)]
+    A, B = enc_lazy(a, 0), enc_lazy(b, 1)
+    out = bg.field_op(2, 2, A, B)
+    assert all(w < 2 * p for w in u32_ints(out))
+    assert dec(out) == [O.fq2_mul(x, y) for x, y in zip(a, b)]
