@@ -146,21 +146,30 @@ def traffic_from_profile() -> dict | None:
     return None
 
 
+# MI355X's published VALU issue rate: 256 CUs x 64 lanes x 2.4 GHz peak engine clock, one
+# 32-bit multiply(-add) per lane per clock
+SPEC_MUL_RATE = 256 * 64 * 2.4e9
+
+
 def roofline(standalone: dict, peak_mul: float) -> dict:
     """Roofline of the G1 accumulation kernel (the proof's dominant kernel) from its
     standalone average launch duration (HIP events on the library stream): one mixed add
-    (10 Fq products x 136 multiplies) per sorted (bucket, point) entry."""
+    (10 Fq products x 136 multiplies) per sorted (bucket, point) entry.  `peak` is the
+    published issue rate; `peak_measured` is the v_mad_u64_u32 probe on this machine."""
     k = standalone["msm_g1_accumulate"]
     entries = bendg.msm_plan_info(k["n"], 1)["entries"]
     work = entries * FQ_PER_MADD[1] * MULS_PER_FQ
     achieved = work / (k["avg_ms"] * 1e-3) / 1e12
+    spec = SPEC_MUL_RATE / 1e12
     tr = traffic_from_profile()
     return {"bound": "valu", "kernel": "k_accumulate<G1> (msm_g1_accumulate)", "achieved": round(achieved, 3),
-            "peak": round(peak_mul / 1e12, 3), "unit": "T int32-mul/s", "frac": round(achieved / (peak_mul / 1e12), 4),
+            "peak": round(spec, 3), "unit": "T int32-mul/s", "frac": round(achieved / spec, 4),
+            "peak_measured": round(peak_mul / 1e12, 3), "frac_of_measured": round(achieved / (peak_mul / 1e12), 4),
             "traffic": tr["bytes"] if tr else None, "traffic_source": tr["source"] if tr else None,
             "algorithmic_bytes": entries * (8 + 64), "work_per_launch": work, "points_per_launch": k["n"],
             "entries_per_launch": entries, "avg_launch_ms": k["avg_ms"],
-            "peak_note": "measured v_mad_u64_u32 issue rate (spec VALU rate 256 CU x 64 lanes x 2.4 GHz = 39.3 T/s)"}
+            "peak_note": "published VALU issue rate 256 CU x 64 lanes x 2.4 GHz; peak_measured: "
+                         "v_mad_u64_u32 issue-rate probe (16 independent chains per lane, 4 waves per SIMD)"}
 
 
 def cpu_baseline(key: Path) -> dict:
@@ -404,8 +413,10 @@ def primitives_mode(a) -> dict:
     hip = Hip()
     L = bendg.lib()
     out = {}
-    peak = probe_peak() / 1e12  # T mul/s
-    out["mul_peak_T_per_s"] = round(peak, 3)
+    peak = probe_peak() / 1e12  # T mul/s, measured
+    spec = SPEC_MUL_RATE / 1e12
+    out["mul_peak_T_per_s"] = round(spec, 3)
+    out["mul_peak_measured_T_per_s"] = round(peak, 3)
     rng = np.random.default_rng(a.seed)
     R = 21888242871839275222246405745257275088548364400416034343698204186575808495617
     flags = bendg.FMT_PACKED | bendg.MEM_DEVICE
@@ -450,8 +461,8 @@ def primitives_mode(a) -> dict:
         fqm = n * w * madd + w * (1 << (c - 1)) * 2 * add + 254 * dbl + w * add
         tmul = fqm * MULS_PER_FQ / (ms * 1e-3) / 1e12
         out[f"msm_g{group}_2^{lg}"] = {"ms": round(ms, 3), "points_per_s": round(n / ms * 1e3, 0), "c": c,
-                                       "algorithmic_T_mul_per_s": round(tmul, 2), "valu_frac": round(tmul / peak, 3),
-                                       "closed_form_ok": ok}
+                                       "algorithmic_T_mul_per_s": round(tmul, 2), "valu_frac": round(tmul / spec, 3),
+                                       "valu_frac_of_measured": round(tmul / peak, 3), "closed_form_ok": ok}
         log(f"  MSM G{group} 2^{lg}: {ms:.3f} ms, closed form {'ok' if ok else 'MISMATCH'}")
         for p in (dp, ds, dout):
             hip.free(p)
@@ -479,13 +490,13 @@ def primitives_mode(a) -> dict:
         byts = 64 * n
         # both roofs for the transform: algorithmic HBM bytes (one 32 B read and write per
         # element) against 8 TB/s, and (n/2) log2 n Fr products x 136 multiplies against the
-        # measured multiply peak
+        # published multiply issue rate (and the measured one)
         gbps = byts / res["fft"] / 1e6
         tmul = (n // 2) * lg * MULS_PER_FQ / (res["fft"] * 1e-3) / 1e12
         out[f"ntt_2^{lg}"] = {"fft_ms": round(res["fft"], 3), "ifft_ms": round(res["ifft"], 3),
                               "fft_GBps_algorithmic": round(gbps, 1), "hbm_frac": round(gbps / 8000.0, 4),
-                              "fft_T_mul_per_s": round(tmul, 2), "valu_frac": round(tmul / peak, 3),
-                              "round_trip_ok": ok}
+                              "fft_T_mul_per_s": round(tmul, 2), "valu_frac": round(tmul / spec, 3),
+                              "valu_frac_of_measured": round(tmul / peak, 3), "round_trip_ok": ok}
         log(f"  NTT 2^{lg}: fft {res['fft']:.3f} ms, ifft {res['ifft']:.3f} ms, round trip {'ok' if ok else 'BAD'}")
         if not ok:
             raise SystemExit("NTT round trip failed")
@@ -515,7 +526,7 @@ def roofline_mode(a) -> dict:
     peak = probe_peak()
     alone = standalone_msm_pass(key, hdr, 5)
     return {"metric": "roofline of the G1 accumulation kernel", "value": roofline(alone, peak)["frac"],
-            "unit": "fraction of measured peak", "roofline": roofline(alone, peak),
+            "unit": "fraction of the published VALU issue rate", "roofline": roofline(alone, peak),
             "kernel_ms_standalone": {k: v["avg_ms"] for k, v in alone.items()}, "copy_bytes": nb}
 
 
