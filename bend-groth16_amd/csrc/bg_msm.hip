@@ -70,19 +70,18 @@ static void finish_plan(MsmPlan& p, int group) {
   p.K = std::min<uint32_t>(kmax, p.G);
   p.H = p.G / p.K;
   p.entries = (uint64_t)p.Wd * p.n;
-  // entries per accumulation thread: about 40 (G1) / 20 (G2), rounded so that the launch
-  // fills a whole number of device rounds (threads resident at once), which leaves no tail
-  // of a few waves.  Longer slices leave fewer partial runs to merge: for G1, one round of
-  // ~40-60 entries instead of two of ~20 costs the accumulation ~5 % and halves the first
-  // merge level (2^18 proof 4.74 -> 4.67 ms, windowed 2^19 MSM 1.62 -> 1.58 ms).
+  // entries per accumulation thread: about 40, rounded so that the launch fills a whole
+  // number of device rounds (threads resident at once), which leaves no tail of a few
+  // waves.  Longer slices leave fewer partial runs to merge: one round of ~40-60 entries
+  // instead of two of ~20 costs the accumulation ~5 % and halves the first merge level
+  // (2^18 proof 4.74 -> 4.67 ms, G1 MSM 2^20 3.08 -> 2.84 ms, G2 MSM 2^20 7.01 -> 6.54 ms).
   uint32_t lenv = (uint32_t)tune_env(group == 1 ? "BG16_MSM_L" : "BG16_MSM_L_G2");
   uint64_t L;
   if (lenv) {
     L = lenv;
   } else {
     const uint64_t cap = accumulate_threads_per_round(group);
-    const uint64_t target = group == 1 ? 40 : 20;
-    const uint64_t k = std::max<uint64_t>(1, p.entries / (target * cap));
+    const uint64_t k = std::max<uint64_t>(1, p.entries / (40 * cap));
     L = (p.entries + k * cap - 1) / (k * cap);
     // G2 MSMs of less than a device round: the rule above gives 2-3 entries a slice, and
     // the merge tree then adds the runs of a bucket in several dependent rounds of G2
