@@ -234,3 +234,22 @@ def test_msm_reduction_block_override(bg, monkeypatch, group, kval):
     got = _msm(group, pts, sc)
     monkeypatch.delenv("BG16_MSM_K")
     assert np.array_equal(got, _oracle(group, pts, sc))
+
+
+@pytest.mark.parametrize("group", [1, 2], ids=["G1", "G2"])
+def test_msm_plan_slices(bg, group):
+    """Accumulation slicing of the windowed planner: every sorted entry belongs to exactly
+    one slice of 2..64 entries; G2 MSMs of 2^16 entries or more take slices of at least 8;
+    MSMs of 2^24 entries or more take slices of at least 20 (one device round holds far
+    fewer threads than entries / 20)."""
+    for log2n in (4, 10, 14, 16, 18, 20, 22):
+        n = 1 << log2n
+        info = bg.msm_plan_info(n, group)
+        E, L, T = info["entries"], info["L"], info["acc_threads"]
+        assert E == info["windows"] * n, (log2n, info)
+        assert 2 <= L <= 64, (log2n, info)
+        assert T == -(-E // L), (log2n, info)
+        if group == 2 and E >= 1 << 16:
+            assert L >= 8, (log2n, info)
+        if E >= 1 << 24:
+            assert L >= 20, (log2n, info)
