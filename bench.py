@@ -6,9 +6,13 @@ SquareChain with N = 2^K constraints (x_{i+1} = x_i^2), domain d = 2^(K+1), prov
 from this library's seeded trapdoor setup (synthetic, seed 42), key resident in HBM and
 the packed witness already resident in HBM when the timed region starts.  A step is a
 full prove: evaluation vectors, qap_h (7 NTTs of 2^(K+1)), the five MSMs (A, B1, H, L in
-G1; B2 in G2) and the proof combination; the output is the affine proof (A, B, C).  Outside
-the timed region the proof is also made from the host witness (PCIe-inclusive time) and
-checked with the pairing equation against vk.bin (host verifier, "proof_verified").
+G1; B2 in G2) and the proof combination; the output is the affine proof (A, B, C).  After
+the timed region the same number of proofs is made from the host witness (FORMAT.md words,
+the PCIe-inclusive time, reported beside `value`), the proof is checked with the pairing
+equation against vk.bin (host verifier, "proof_verified"), and the line carries the
+batched mix (64 proofs, K uniform in 10..18, own keys: engine vs one at a time, every
+proof bit-equal) and the CPU baseline (the C oracle's proof of the same key on the host
+cores, bit-equal to the GPU proof, plus a one-thread run).
 
 Prints one JSON line (rank 0).  With --gpus N > 1 it exits: this build measures one GPU.
 
@@ -99,6 +103,22 @@ MULS_PER_FQ = 136
 FQ_PER_MADD = {1: 10, 2: 28}   # madd-2008-s: 8M + 2S (G2: Karatsuba 3 / complex square 2)
 
 
+R_MOD = 21888242871839275222246405745257275088548364400416034343698204186575808495617
+
+
+def uniform_fr(rng: np.random.Generator, n: int) -> np.ndarray:
+    """(n, 8) u32 canonical scalars uniform in [0, r): uniform 256-bit words with the top
+    word drawn below r's, redrawn where the whole value lands at or above r."""
+    w = rng.integers(0, 2**64, size=(n, 4), dtype=np.uint64)
+    top = R_MOD >> 192
+    w[:, 3] = rng.integers(0, top + 1, size=n, dtype=np.uint64)
+    for i in np.nonzero(w[:, 3] == top)[0]:
+        while sum(int(w[i, j]) << (64 * j) for j in range(4)) >= R_MOD:
+            w[i] = rng.integers(0, 2**64, size=4, dtype=np.uint64)
+            w[i, 3] = rng.integers(0, top + 1, dtype=np.uint64)
+    return w.view(np.uint32).reshape(n, 8)
+
+
 def msm_sizes(hdr):
     nc, ni, nw, lg, na, nb, nh, nl = hdr
     return {1: [na, nb, nh, nl], 2: [nb]}
@@ -112,8 +132,7 @@ def standalone_msm_pass(key: Path, hdr, reps: int) -> dict:
     out = {}
     for g, fname, n in ((1, "pk_h.bin", hdr[6]), (2, "pk_b2.bin", hdr[5])):
         pts = bendg.pack16(np.fromfile(key / fname, dtype=np.uint32).reshape(-1, 16)).reshape(n, -1)
-        sc = rng.integers(0, 2**32, size=(n, 8), dtype=np.uint32)
-        sc[:, 7] &= np.uint32(0x0FFFFFFF)
+        sc = uniform_fr(rng, n)
         f = bendg.msm_g1 if g == 1 else bendg.msm_g2
         f(pts, sc)  # warm-up (allocations)
         bendg.timing_reset()
@@ -172,18 +191,45 @@ def roofline(standalone: dict, peak_mul: float) -> dict:
                          "v_mad_u64_u32 issue-rate probe (16 independent chains per lane, 4 waves per SIMD)"}
 
 
-def cpu_baseline(key: Path) -> dict:
-    """The C oracle's full prove of the same instance on the host cores (one proof)."""
+def cpu_baseline(key: Path, gpu_proof: np.ndarray, keys_dir: Path) -> dict:
+    """The C oracle's full prove of the same instance on all host cores (one proof),
+    checked bit-equal to the GPU proof; then the same on one thread.  The one-thread run
+    is bounded: the full proof when the all-core time times the core count stays under a
+    minute, else one proof of the 2^16 SquareChain (named in its sample)."""
     sys.path.insert(0, str(ROOT / "tests"))
     from oracle import oracle as O  # cpu_baseline leg only
     from instance import load
     inst = load(key)
+    cores = O.threads()
     t = time.perf_counter()
-    O.prove(inst)
+    A, B, C, _ = O.prove(inst)
     ms = (time.perf_counter() - t) * 1e3
-    return {"value": round(ms, 1), "unit": "ms/proof", "cores": O.threads(), "kind": "port",
-            "sample": "1 full SquareChain proof of the same key by the C oracle (OpenMP, reference-shaped "
-                      "Pippenger c=8/10 unsigned windows, radix-2 NTT)"}
+    got = np.ascontiguousarray(gpu_proof, dtype=np.uint32).reshape(-1).view(np.uint64)
+    equal = bool(np.array_equal(got[:8], A) and np.array_equal(got[8:24], B) and np.array_equal(got[24:32], C))
+    if not equal:
+        raise SystemExit("the CPU baseline's proof differs from the GPU proof")
+    one = {}
+    O.set_threads(1)
+    try:
+        if ms * cores <= 60e3:
+            t = time.perf_counter()
+            O.prove(inst)
+            one = {"value": round((time.perf_counter() - t) * 1e3, 1), "unit": "ms/proof", "cores": 1,
+                   "sample": "1 full SquareChain proof of the same key, one thread"}
+        else:
+            k16 = load(ensure_key(16, 42, keys_dir))
+            t = time.perf_counter()
+            O.prove(k16)
+            one = {"value": round((time.perf_counter() - t) * 1e3, 1), "unit": "ms/proof (SquareChain 2^16)",
+                   "cores": 1, "sample": "1 full SquareChain 2^16 proof, one thread (the 2^18 proof on one "
+                                         "thread would exceed the bench's time bound)"}
+    finally:
+        O.set_threads(cores)
+    return {"value": round(ms, 1), "unit": "ms/proof", "cores": cores, "kind": "port",
+            "equal_to_gpu_proof": equal,
+            "sample": "1 full SquareChain proof of the same key by the C oracle (OpenMP over all host cores, "
+                      "reference-shaped Pippenger c=8/10 unsigned windows, radix-2 NTT), bit-equal to the GPU proof",
+            "one_thread": one}
 
 
 # ------------------------------------------------------------------ engine / batch modes
@@ -212,29 +258,37 @@ def witness_files(d: Path):
     return z, rs
 
 
-def batch_mode(a) -> dict:
+def batched_mix(a, hip, slots_list) -> dict:
+    """64 SquareChain proofs with K uniform in 10..18 (seeded), each with its own key and
+    witness (witnesses resident in HBM): one proof at a time on the single-proof path vs
+    all 64 submitted at once to the proving engine; every engine proof must equal the
+    single proof.  Also reports the single path's time by K and the share of it spent in
+    the K >= 17 proofs: one such proof already keeps the device busy (the same-key burst
+    at 2^18 gains nothing from batching), so 1 / share bounds what any batching of this
+    mix can gain."""
     rng = np.random.default_rng(a.seed)
     ks = [int(k) for k in rng.integers(10, 19, size=64)]
     base = Path(a.keys) / "batch"
+    t = time.perf_counter()
     dirs = [ensure_key(k, 1000 + i, base) for i, k in enumerate(ks)]
-    bendg.init(0)
-    hip = Hip()
     pks = [bendg.ProvingKey(d) for d in dirs]
     wz = [witness_files(d) for d in dirs]
     dzs = [hip.upload(z) for z, _ in wz]
-    log(f"64 keys loaded, K = {ks}")
+    log(f"64 keys ready in {time.perf_counter() - t:.1f} s, K = {ks}")
     # one proof at a time (the single-proof path), after one warm proof per key
     ref = [pk.prove_device(dz, rs)[0] for pk, dz, (_, rs) in zip(pks, dzs, wz)]
     hip.sync()
+    per = []
     t0 = time.perf_counter()
     for pk, dz, (_, rs), r in zip(pks, dzs, wz, ref):
+        s0 = time.perf_counter()
         p, _ = pk.prove_device(dz, rs)
+        per.append(time.perf_counter() - s0)
         if not np.array_equal(p, r):
             raise SystemExit("single proof not reproducible")
     single_s = time.perf_counter() - t0
-    # the proving engine: all 64 submitted at once, slots prove concurrently
     res = {}
-    for slots in a.slots:
+    for slots in slots_list:
         eng = bendg.Engine(slots=slots)
         for t in [eng.submit(pk, dz, rs) for pk, dz, (_, rs) in zip(pks, dzs, wz)]:
             eng.wait(t)  # warm: every slot's work arrays at their largest
@@ -252,15 +306,34 @@ def batch_mode(a) -> dict:
                       "p50_ms": round(float(np.percentile(lat, 50)), 2),
                       "p99_ms": round(float(np.percentile(lat, 99)), 2)}
         log(f"engine slots={slots}: {res[slots]}")
-    best = max(res.values(), key=lambda v: v["proofs_per_s"])
-    single_rate = 64 / single_s
-    msm = batch_msm_256(a, hip)
     for pk in pks:
         pk.close()
+    for dz in dzs:
+        hip.free(dz)
+    best = max(res.values(), key=lambda v: v["proofs_per_s"])
+    single_rate = 64 / single_s
+    by_k = {}
+    for k, dt in zip(ks, per):
+        by_k.setdefault(k, []).append(dt * 1e3)
+    big = sum(dt for k, dt in zip(ks, per) if k >= 17)
+    share = big / sum(per)
+    return {"proofs": 64, "engine_proofs_per_s": best["proofs_per_s"], "single_proofs_per_s": round(single_rate, 2),
+            "vs_single": round(best["proofs_per_s"] / single_rate, 3),
+            "engine": {str(k): v for k, v in res.items()}, "ks": ks,
+            "single_ms_by_k": {str(k): round(float(np.median(v)), 3) for k, v in sorted(by_k.items())},
+            "single_time_share_k17_18": round(share, 3), "ceiling_vs_single": round(1.0 / share, 3),
+            "all_equal_single": True}
+
+
+def batch_mode(a) -> dict:
+    bendg.init(0)
+    hip = Hip()
+    mix = batched_mix(a, hip, a.slots)
+    msm = batch_msm_256(a, hip)
     return {"metric": "batched proofs/s, 64 SquareChain proofs K~U{10..18} (bit-exact)",
-            "value": best["proofs_per_s"], "unit": "proofs/s", "n_gpus": 1, "higher_is_better": True,
-            "single_proofs_per_s": round(single_rate, 2), "batched_vs_single": round(best["proofs_per_s"] / single_rate, 3),
-            "engine": {str(k): v for k, v in res.items()}, "ks": ks, "dtype": "u32 (exact BN254 integer arithmetic)",
+            "value": mix["engine_proofs_per_s"], "unit": "proofs/s", "n_gpus": 1, "higher_is_better": True,
+            "single_proofs_per_s": mix["single_proofs_per_s"], "batched_vs_single": mix["vs_single"],
+            "mix": mix, "dtype": "u32 (exact BN254 integer arithmetic)",
             "data": "synthetic (own trapdoor setup per proof, seeds 1000+i)", "msm_256": msm,
             "config": {"workload": "64 proofs, K uniform in 10..18, own key and witness each, witnesses resident in HBM",
                        "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES")}}
@@ -275,8 +348,7 @@ def batch_msm_256(a, hip) -> dict:
     pts = bendg.gen_bases(1, 77, pool_n)
     dpts = hip.upload(pts)
     tot = sum(ns)
-    sc = rng.integers(0, 2**32, size=(tot, 8), dtype=np.uint32)
-    sc[:, 7] &= np.uint32(0x1FFFFFFF)
+    sc = uniform_fr(rng, tot)
     dsc = hip.upload(sc)
     offs = np.concatenate([[0], np.cumsum(ns)[:-1]]).astype(np.uint64)
     L = bendg.lib()
@@ -425,8 +497,7 @@ def primitives_mode(a) -> dict:
         t = time.perf_counter()
         pts, ks = bendg.gen_bases(group, 100 + lg, n, with_k=True)
         log(f"G{group} 2^{lg}: bases in {time.perf_counter() - t:.1f} s")
-        sc = rng.integers(0, 2**32, size=(n, 8), dtype=np.uint32)
-        sc[:, 7] &= np.uint32(0x1FFFFFFF)
+        sc = uniform_fr(rng, n)
         dp, ds = hip.upload(pts), hip.upload(sc)
         pw = 16 if group == 1 else 32
         dout = hip.alloc(pw * 4)
@@ -470,8 +541,7 @@ def primitives_mode(a) -> dict:
             raise SystemExit("primitive MSM differs from its closed form")
     for lg in (20, 24):
         n = 1 << lg
-        x = rng.integers(0, 2**32, size=(n, 8), dtype=np.uint32)
-        x[:, 7] &= np.uint32(0x0FFFFFFF)
+        x = uniform_fr(rng, n)
         d = hip.upload(x)
         res = {}
         for name, dirn in (("fft", bendg.NTT_FFT), ("ifft", bendg.NTT_IFFT)):
@@ -539,22 +609,23 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--keys", default=os.environ.get("BG16_KEYS", "/tmp/bg16_keys"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mix", action="store_true", help="single mode: skip the 64-proof batched mix")
     ap.add_argument("--mode", default="single", choices=["single", "batch", "serve", "primitives", "roofline"])
     ap.add_argument("--slots", type=lambda v: [int(x) for x in v.split(",")], default=[4, 8],
                     help="engine slots (batch: comma list to sweep)")
     ap.add_argument("--rates", type=lambda v: [float(x) for x in v.split(",")], default=[100.0, 200.0, 300.0],
                     help="serve: arrival rates in proofs/s")
     ap.add_argument("--duration", type=float, default=20.0, help="serve: seconds per arrival rate")
-    ap.add_argument("--hw-queues", type=int, default=0,
-                    help="HIP hardware queues for this process (each proof in flight uses four streams); "
-                         "0 keeps the environment's value for the single mode and uses 16 for the others")
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="HIP hardware queues for this process (each proof in flight uses four streams, "
+                         "so the engine's slots need more than the default 4); 0 keeps the environment's value")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
         return 2
     # must precede the first HIP call of the process
-    hwq = a.hw_queues or (16 if a.mode != "single" else 0)
+    hwq = a.hw_queues
     if hwq:
         os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, max(1, hwq)))
     if a.mode != "single":
@@ -582,20 +653,8 @@ def main() -> int:
         p, ph = pk.prove_device(dz, rs)
         ref = p if ref is None else ref
         log(f"warmup {i}: " + " ".join(f"{k}={v:.2f}" for k, v in ph.items()))
-    # host-buffer (PCIe-inclusive) proofs for reference: the witness as FORMAT.md words in
-    # pageable host memory, packed into pinned staging and copied in by the call; one
-    # warm-up (it sizes the staging buffer), then the median of five
-    p_host, _ = pk.prove(z16, rs16, fmt="words16")
-    pcie_walls = []
-    for _ in range(5):
-        p_h, ph_host = pk.prove(z16, rs16, fmt="words16")
-        pcie_walls.append(ph_host["total"])
-        if not np.array_equal(p_h, p_host):
-            log("host-witness proofs differ between calls")
-            return 1
-    pcie_ms = float(np.median(pcie_walls))
-    bendg.set_timing(True)
-    bendg.timing_reset()
+    # timed region: `steps` proofs back to back, key and packed witness resident in HBM,
+    # no per-kernel event timing
     hip.sync()
     t0 = time.perf_counter()
     walls = []
@@ -610,13 +669,34 @@ def main() -> int:
             return 1
     hip.sync()
     elapsed = time.perf_counter() - t0
-    timing = bendg.timing_report()
-    bendg.set_timing(False)
+    ms = elapsed * 1e3 / a.steps
+    # the same `steps` proofs with the witness handed over in host memory as FORMAT.md
+    # words (the reference's boundary: packed into pinned staging and copied in by the
+    # call); one untimed call first sizes the staging buffer.  Reported beside `value`.
+    p_host, _ = pk.prove(z16, rs16, fmt="words16")
+    hip.sync()
+    t0 = time.perf_counter()
+    pcie_walls = []
+    for _ in range(a.steps):
+        s = time.perf_counter()
+        p_h, _ = pk.prove(z16, rs16, fmt="words16")
+        pcie_walls.append((time.perf_counter() - s) * 1e3)
+        if not np.array_equal(p_h, p_host):
+            log("host-witness proofs differ between calls")
+            return 1
+    pcie_ms = (time.perf_counter() - t0) * 1e3 / a.steps
     if not np.array_equal(bendg.pack16(p_host.reshape(-1, 16)).reshape(-1), ref):
         log("host-witness proof differs from the device-witness proof")
         return 1
-    ms = elapsed * 1e3 / a.steps
-    per_proof = {k: round(v[0] * v[1] / a.steps, 4) for k, v in sorted(timing.items(), key=lambda kv: -kv[1][0] * kv[1][1])}
+    # per-kernel HIP-event times (overlapped, per proof) from separate proofs
+    bendg.set_timing(True)
+    bendg.timing_reset()
+    nt = max(3, min(a.steps, 10))
+    for _ in range(nt):
+        pk.prove_device(dz, rs)
+    timing = bendg.timing_report()
+    bendg.set_timing(False)
+    per_proof = {k: round(v[0] * v[1] / nt, 4) for k, v in sorted(timing.items(), key=lambda kv: -kv[1][0] * kv[1][1])}
     peak = probe_peak()
     log("standalone MSM pass")
     alone = standalone_msm_pass(key, hdr, 5)
@@ -634,6 +714,10 @@ def main() -> int:
     log("engine (concurrent proofs of this key)")
     batched = engine_rate(pk, dz, rs, 32, 4, ref)
     batched["vs_single"] = round(batched["proofs_per_s"] / (1e3 / ms), 3)
+    mix = None
+    if not a.no_mix:
+        log("batched mix: 64 proofs, K uniform in 10..18, own keys")
+        mix = batched_mix(a, hip, [4])
     line = {
         "metric": METRIC, "value": round(ms, 3), "unit": "ms/proof", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(ms, 3), "higher_is_better": False, "scaling": "weak",
@@ -645,16 +729,19 @@ def main() -> int:
         "latency_ms": {"p50": round(float(np.percentile(walls, 50)), 3),
                        "p99": round(float(np.percentile(walls, 99)), 3)},
         "pcie_inclusive_ms": round(pcie_ms, 3),
+        "pcie_inclusive_latency_ms": {"p50": round(float(np.percentile(pcie_walls, 50)), 3),
+                                      "p99": round(float(np.percentile(pcie_walls, 99)), 3)},
         "proof_verified": verified,
         "key_load_ms": round(load_ms, 1),
         "kernel_ms_per_proof_overlapped": per_proof,
         "kernel_ms_standalone": {k: v["avg_ms"] for k, v in alone.items()},
         "roofline": roof,
         "batched_same_key": batched,
+        "batched_mix": mix,
     }
     if not a.no_cpu_baseline:
         log("cpu baseline (C oracle, one proof)")
-        line["cpu_baseline"] = cpu_baseline(key)
+        line["cpu_baseline"] = cpu_baseline(key, ref, Path(a.keys))
     print(json.dumps(line), flush=True)
     hip.free(dz)
     pk.close()

@@ -63,8 +63,10 @@ class ProvingKey:
         return out, dict(zip(PHASES, (float(x) for x in t)))
 
     def close(self):
+        """Frees the key.  Raises (and keeps the key loaded) while a proof of it is queued
+        or running in an engine."""
         if self._h:
-            lib().bg_pk_free(self._h)
+            _check(lib().bg_pk_free(self._h))
             self._h = None
 
     def __del__(self):
@@ -102,11 +104,13 @@ class Engine:
                 raise ValueError("witness / rs sizes do not match the key")
             flag = FMT_WORDS16 if fmt == "words16" else FMT_PACKED
             _check(lib().bg_engine_submit(self._h, pk.handle, _p(w), _p(r), flag, ctypes.byref(t)))
-        self._keep[t.value] = 16 if (fmt == "words16" and not isinstance(witness, int)) else 8
+        # the key object stays referenced until the proof is collected, so it cannot be
+        # garbage-collected (and freed) while the engine still uses it
+        self._keep[t.value] = (16 if (fmt == "words16" and not isinstance(witness, int)) else 8, pk)
         return t.value
 
     def wait(self, ticket: int):
-        ew = self._keep.pop(ticket, 8)
+        ew, _pk = self._keep.pop(ticket, (8, None))
         out = np.zeros(8 * ew, dtype=np.uint32)
         tm = np.zeros(3, dtype=np.float64)
         _check(lib().bg_engine_wait(self._h, ticket, _p(out), _p(tm)))
@@ -132,9 +136,14 @@ class Engine:
 def prove_batch(pks, witnesses, rss, fmt: str = "words16", slots: int = 4) -> list:
     """``bg_prove_batch``: one proof per (key, witness, rs), returned in order."""
     n = len(pks)
+    if len(witnesses) != n or len(rss) != n:
+        raise ValueError("prove_batch: one witness and one rs per key")
     ew = 16 if fmt == "words16" else 8
     ws = [np.ascontiguousarray(w, dtype=np.uint32).reshape(-1) for w in witnesses]
     rr = [np.ascontiguousarray(r, dtype=np.uint32).reshape(-1) for r in rss]
+    for pk, w, r in zip(pks, ws, rr):
+        if w.size != (pk.header[1] + pk.header[2]) * ew or r.size != 2 * ew:
+            raise ValueError("prove_batch: witness / rs sizes do not match the key")
     outs = [np.zeros(8 * ew, dtype=np.uint32) for _ in range(n)]
     arr = lambda xs: ctypes.cast((ctypes.c_void_p * n)(*[x.ctypes.data for x in xs]), ctypes.c_void_p)  # noqa: E731
     kp = ctypes.cast((ctypes.c_void_p * n)(*[pk.handle.value for pk in pks]), ctypes.c_void_p)

@@ -189,6 +189,7 @@ static void worker_main(bg_engine* e, int slot_id) {
         r->status = rc < 0 ? rc : 0;
         r->err = msg;
         e->completed++;
+        pk_release(r->pk);
       }
       // actual footprint after the run (grow-only buffers)
       if (slot) e->reserved[slot_id] = std::max(e->reserved[slot_id], slot->device_bytes());
@@ -231,6 +232,7 @@ extern "C" int bg_engine_submit(bg_engine* e, bg_pk* pk, const uint32_t* witness
     BG_REQUIRE(!(dev && words16), BG_ERR_ARG, "bg_engine_submit: a device witness must be packed");
     uint32_t hdr[8];
     BG_REQUIRE(bg_pk_header(pk, hdr) == 0, BG_ERR_ARG, "bg_engine_submit: bad key");
+    check_host_inputs(witness, rs, flags, (uint64_t)hdr[1] + hdr[2]);
     auto r = std::make_unique<Request>();
     r->pk = pk;
     r->flags = flags;
@@ -248,6 +250,7 @@ extern "C" int bg_engine_submit(bg_engine* e, bg_pk* pk, const uint32_t* witness
                (unsigned long long)r->need, (unsigned long long)e->budget);
     r->ticket = e->next_ticket++;
     r->t_submit = Clock::now();
+    pk_acquire(pk);  // released when the proof is done: bg_pk_free refuses the key until then
     *ticket = r->ticket;
     e->queue.push_back(r.get());
     e->all[r->ticket] = std::move(r);

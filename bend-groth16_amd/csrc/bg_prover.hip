@@ -13,6 +13,7 @@
 #include "bg_ntt.hpp"
 #include "bg_field.hpp"
 #include "bg_host_ec.hpp"
+#include <atomic>
 #include <chrono>
 #include <memory>
 #include <fstream>
@@ -56,6 +57,62 @@ static std::vector<uint32_t> packed_file(const std::string& path, size_t nwords1
   return out;
 }
 
+// Field moduli as little-endian 32-bit words: FORMAT.md stores x * 2^256 mod p, so every
+// stored element is below its modulus.
+static const uint32_t* const MOD_R = FrP::p;
+static const uint32_t* const MOD_Q = FqP::p;
+
+static bool below(const uint32_t* x, const uint32_t* p) {
+  for (int i = 7; i >= 0; i--) {
+    if (x[i] != p[i]) return x[i] < p[i];
+  }
+  return false;
+}
+
+// Index of the first of n packed elements (8 words each) that is not below p, or n.
+static uint64_t first_unreduced(const uint32_t* x, uint64_t n, const uint32_t* p) {
+  for (uint64_t i = 0; i < n; i++)
+    if (!below(x + 8 * i, p)) return i;
+  return n;
+}
+
+// y^2 = x^3 + b for packed affine Montgomery points (all-zero = infinity, accepted):
+// G1 b = 3, G2 b = 3 / (9 + u) (the D-type twist).  Coordinates are checked below q
+// first.  Returns the index of the first bad point, or n.
+static uint64_t first_off_curve(int group, const uint32_t* pts, uint64_t n) {
+  const int ew = group == 1 ? 8 : 16, pw = 2 * ew;
+  const bgh::Fq b1 = bgh::Fq::to_mont(bgh::Fq::raw(3));
+  const bgh::Fq2 xi = {bgh::Fq::to_mont(bgh::Fq::raw(9)), bgh::Fq::to_mont(bgh::Fq::raw(1))};
+  const bgh::Fq2 b2 = xi.inv() * bgh::Fq2{b1, bgh::Fq::zero()};
+  const uint64_t chunk = 1u << 13;
+  const int nch = (int)((n + chunk - 1) / chunk);
+  std::vector<uint64_t> bad(std::max(nch, 1), n);
+  host_parallel(nch, [&](int t) {
+    for (uint64_t i = (uint64_t)t * chunk; i < std::min<uint64_t>(n, (uint64_t)(t + 1) * chunk); i++) {
+      const uint32_t* q = pts + (uint64_t)pw * i;
+      bool zero = true;
+      for (int k = 0; k < pw; k++) zero &= q[k] == 0;
+      if (zero) continue;
+      bool ok = true;
+      for (int k = 0; k < pw / 8; k++) ok &= below(q + 8 * k, MOD_Q);
+      if (ok) {
+        if (group == 1) {
+          const bgh::Fq x = bgh::Fq::from_u32(q), y = bgh::Fq::from_u32(q + 8);
+          ok = y.sqr() == x.sqr() * x + b1;
+        } else {
+          const bgh::Fq2 x = bgh::Fq2::from_u32(q), y = bgh::Fq2::from_u32(q + 16);
+          ok = y.sqr() == x.sqr() * x + b2;
+        }
+      }
+      if (!ok) {
+        bad[t] = i;
+        return;
+      }
+    }
+  });
+  return *std::min_element(bad.begin(), bad.end());
+}
+
 struct Csr {
   uint32_t nnz = 0;
   DevArray offs, cols, coef;
@@ -78,6 +135,8 @@ struct bg_pk {
   // slot used by bg_prove (created on first use; one bg_prove call per key at a time)
   std::unique_ptr<bg::ProveSlot> own;
   std::mutex own_mu;
+  // proofs of this key queued in an engine or running (bg_pk_free refuses while > 0)
+  std::atomic<int> inflight{0};
 };
 
 namespace bg {
@@ -168,6 +227,7 @@ static HostCsr read_csr(const std::string& path, uint32_t nc, uint64_t nz) {
     BG_REQUIRE(h.cols[k] < nz, BG_ERR_IO, "%s: column %u out of range (%llu wires)", path.c_str(), h.cols[k],
                (unsigned long long)nz);
     pack_words(e + 17ull * k + 1, &h.coef[8ull * k], 8, path.c_str());
+    BG_REQUIRE(below(&h.coef[8ull * k], MOD_R), BG_ERR_IO, "%s: coefficient %u is not below r", path.c_str(), k);
   }
   return h;
 }
@@ -193,6 +253,9 @@ extern "C" int bg_pk_load(const char* dir, bg_pk** out) {
     HostCsr cc = read_csr(d + "/r1cs_c.bin", nc, nz);
     auto g1 = packed_file(d + "/pk_g1.bin", 3 * 32, "pk_g1.bin");
     auto g2 = packed_file(d + "/pk_g2.bin", 2 * 64, "pk_g2.bin");
+    BG_REQUIRE(first_off_curve(1, g1.data(), 3) == 3, BG_ERR_IO, "%s/pk_g1.bin: a point is not on G1", dir);
+    BG_REQUIRE(first_off_curve(2, g2.data(), 2) == 2, BG_ERR_IO, "%s/pk_g2.bin: a point is not on the twist",
+               dir);
     memcpy(pk->g1, g1.data(), sizeof pk->g1);
     memcpy(pk->g2, g2.data(), sizeof pk->g2);
     struct Q {
@@ -206,7 +269,13 @@ extern "C" int bg_pk_load(const char* dir, bg_pk** out) {
               {"pk_b2.bin", nbq, 64, &pk->pb2, {}},
               {"pk_h.bin", nh, 32, &pk->ph, {}},
               {"pk_l.bin", nl, 32, &pk->pl, {}}};
-    for (auto& q : qs) q.host = packed_file(d + "/" + q.name, q.n * q.w16, q.name);
+    for (auto& q : qs) {
+      q.host = packed_file(d + "/" + q.name, q.n * q.w16, q.name);
+      const int group = q.w16 == 32 ? 1 : 2;
+      const uint64_t bad = first_off_curve(group, q.host.data(), q.n);
+      BG_REQUIRE(bad == q.n, BG_ERR_IO, "%s/%s: point %llu is not on %s", dir, q.name, (unsigned long long)bad,
+                 group == 1 ? "G1" : "the twist");
+    }
 
     // 2. upload into HBM
     Ctx& c = ctx();
@@ -251,6 +320,8 @@ extern "C" int bg_pk_load(const char* dir, bg_pk** out) {
 extern "C" int bg_pk_free(bg_pk* pk) {
   return guard([&] {
     if (!pk) return;
+    const int busy = pk->inflight.load();
+    BG_REQUIRE(busy == 0, BG_ERR_ARG, "bg_pk_free: %d proofs of this key are queued or running", busy);
     {
       std::lock_guard<std::mutex> lk(pk->own_mu);
       pk->own.reset();
@@ -314,6 +385,9 @@ uint64_t ProveSlot::device_bytes() const {
   return t;
 }
 
+void pk_acquire(bg_pk* pk) { pk->inflight.fetch_add(1); }
+void pk_release(bg_pk* pk) { pk->inflight.fetch_sub(1); }
+
 uint64_t prove_bytes_estimate(const bg_pk* pk) {
   const uint32_t ni = pk->hdr[1], nw = pk->hdr[2], lg = pk->hdr[3];
   const uint64_t na = pk->hdr[4], nbq = pk->hdr[5], nh = pk->hdr[6], nl = pk->hdr[7];
@@ -338,6 +412,11 @@ extern "C" int bg_prove(bg_pk* pk, const uint32_t* witness, const uint32_t* rs, 
                         double* t_ms) {
   return guard([&] {
     BG_REQUIRE(pk && witness && rs && proof_out, BG_ERR_ARG, "bg_prove: null argument");
+    pk_acquire(pk);
+    struct Release {
+      bg_pk* k;
+      ~Release() { pk_release(k); }
+    } rel{pk};
     std::lock_guard<std::mutex> lk(pk->own_mu);
     if (!pk->own) pk->own = std::make_unique<ProveSlot>();
     prove_on(pk, *pk->own, witness, rs, proof_out, flags, t_ms);
@@ -359,15 +438,59 @@ static void upload_witness(const uint32_t* witness, int flags, uint64_t nz, uint
     return;
   }
   const uint64_t chunk = 1ull << 14;
-  host_parallel((int)((nz + chunk - 1) / chunk), [&](int t) {
+  const int nch = (int)((nz + chunk - 1) / chunk);
+  // FORMAT.md witness values are Montgomery Fr below r; a 16-bit word with high bits set
+  // or a value at or above r is rejected (checked on the packed staging copy)
+  std::vector<uint64_t> bad(std::max(nch, 1), nz);
+  std::vector<uint32_t> wide(std::max(nch, 1), 0);
+  host_parallel(nch, [&](int t) {
     const uint64_t lo = (uint64_t)t * chunk, hi = std::min<uint64_t>(nz, lo + chunk);
     if (words16) {
-      for (uint64_t i = 8 * lo; i < 8 * hi; i++) zp[i] = witness[2 * i] | (witness[2 * i + 1] << 16);
+      uint32_t hb = 0;
+      for (uint64_t i = 8 * lo; i < 8 * hi; i++) {
+        const uint32_t a = witness[2 * i], b = witness[2 * i + 1];
+        hb |= (a | b) >> 16;
+        zp[i] = a | (b << 16);
+      }
+      wide[t] = hb;
     } else {
       memcpy(zp + 8 * lo, witness + 8 * lo, 32 * (hi - lo));
     }
+    const uint64_t f = first_unreduced(zp + 8 * lo, hi - lo, MOD_R);
+    if (f < hi - lo) bad[t] = lo + f;
   });
+  for (uint32_t w : wide) BG_REQUIRE(w == 0, BG_ERR_ARG, "witness: limb word >= 2^16");
+  const uint64_t b = *std::min_element(bad.begin(), bad.end());
+  BG_REQUIRE(b == nz, BG_ERR_ARG, "witness: element %llu is not below r", (unsigned long long)b);
   BG_HIP(hipMemcpyAsync(z, zp, 32 * nz, hipMemcpyHostToDevice, s));
+}
+
+// r then s (FORMAT.md rs.bin) as packed words, each checked below r
+static void read_rs(const uint32_t* rs, int flags, uint32_t* rs_packed) {
+  if ((flags & 1) == BG_FMT_WORDS16)
+    pack_words(rs, rs_packed, 16, "rs");
+  else
+    memcpy(rs_packed, rs, 64);
+  BG_REQUIRE(first_unreduced(rs_packed, 2, MOD_R) == 2, BG_ERR_ARG, "rs: r or s is not below the modulus");
+}
+
+void check_host_inputs(const uint32_t* witness, const uint32_t* rs, int flags, uint64_t nz) {
+  uint32_t tmp[16];
+  read_rs(rs, flags, tmp);
+  if (flags & BG_MEM_DEVICE) return;
+  const bool words16 = (flags & 1) == BG_FMT_WORDS16;
+  uint32_t e[8];
+  for (uint64_t i = 0; i < nz; i++) {
+    const uint32_t* x = witness + (words16 ? 16 : 8) * i;
+    if (words16) {
+      for (int k = 0; k < 8; k++) {
+        BG_REQUIRE(((x[2 * k] | x[2 * k + 1]) >> 16) == 0, BG_ERR_ARG, "witness: limb word >= 2^16");
+        e[k] = x[2 * k] | (x[2 * k + 1] << 16);
+      }
+      x = e;
+    }
+    BG_REQUIRE(below(x, MOD_R), BG_ERR_ARG, "witness: element %llu is not below r", (unsigned long long)i);
+  }
 }
 
 void prove_on(bg_pk* pk, ProveSlot& sl, const uint32_t* witness, const uint32_t* rs, uint32_t* proof_out, int flags,
@@ -379,6 +502,8 @@ void prove_on(bg_pk* pk, ProveSlot& sl, const uint32_t* witness, const uint32_t*
     BG_REQUIRE(!(dev_witness && words16), BG_ERR_ARG, "bg_prove: a device witness must be packed");
     (void)ctx();  // device current on this thread
     auto t0 = std::chrono::steady_clock::now();
+    uint32_t rs_packed[16];
+    read_rs(rs, flags, rs_packed);
     const uint32_t nc = pk->hdr[0], ni = pk->hdr[1], nw = pk->hdr[2], lg = pk->hdr[3];
     const uint32_t na = pk->hdr[4], nbq = pk->hdr[5], nh = pk->hdr[6], nl = pk->hdr[7];
     const uint64_t d = 1ull << lg, nz = (uint64_t)ni + nw;
@@ -497,11 +622,6 @@ void prove_on(bg_pk* pk, ProveSlot& sl, const uint32_t* witness, const uint32_t*
     BG_HIP(hipEventRecord(ev[11], s0));
 
     // host work that depends only on r, s overlaps with the device
-    uint32_t rs_packed[16];
-    if (words16)
-      pack_words(rs, rs_packed, 16, "rs");
-    else
-      memcpy(rs_packed, rs, sizeof rs_packed);
     uint64_t rc[4], sc[4], rsc[4];
     canon_words(rs_packed, rc);
     canon_words(rs_packed + 8, sc);
@@ -583,6 +703,8 @@ void prove_batch_on(const ProveJob* jobs, int P, ProveSlot& sl) {
     BG_REQUIRE(!((q.flags & BG_MEM_DEVICE) && (q.flags & 1) == BG_FMT_WORDS16), BG_ERR_ARG,
                "prove batch: a device witness must be packed");
     BG_REQUIRE(prove_batchable(pk0, q.pk), BG_ERR_ARG, "prove batch: keys of different shapes");
+    uint32_t tmp[16];
+    read_rs(q.rs, q.flags, tmp);
   }
   (void)ctx();
   const uint32_t nc = pk0->hdr[0], ni = pk0->hdr[1], nw = pk0->hdr[2], lg = pk0->hdr[3];
@@ -654,10 +776,7 @@ void prove_batch_on(const ProveJob* jobs, int P, ProveSlot& sl) {
     const ProveJob& q = jobs[j];
     Comb& c = cb[j];
     uint32_t rs_packed[16];
-    if ((q.flags & 1) == BG_FMT_WORDS16)
-      pack_words(q.rs, rs_packed, 16, "rs");
-    else
-      memcpy(rs_packed, q.rs, sizeof rs_packed);
+    read_rs(q.rs, q.flags, rs_packed);  // checked before any launch
     canon_words(rs_packed, c.rc);
     canon_words(rs_packed + 8, c.sc);
     bgh::Fr x = (bgh::Fr::from_u32(rs_packed) * bgh::Fr::from_u32(rs_packed + 8)).from_mont();

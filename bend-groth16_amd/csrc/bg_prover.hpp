@@ -30,6 +30,14 @@ struct ProveSlot {
 // Device bytes one proof with this key needs in a fresh slot (work arrays + MSM work).
 uint64_t prove_bytes_estimate(const bg_pk* pk);
 
+// Checks a host witness of nz elements (either encoding) and r / s (host) as bg_prove
+// does: 16-bit limbs and every value below r; throws BG_ERR_ARG otherwise.
+void check_host_inputs(const uint32_t* witness, const uint32_t* rs, int flags, uint64_t nz);
+
+// Proofs of a key that are queued or running: bg_pk_free refuses a key while any are.
+void pk_acquire(bg_pk* pk);
+void pk_release(bg_pk* pk);
+
 // Runs one proof on slot sl (blocking until the proof is on the host).
 void prove_on(bg_pk* pk, ProveSlot& sl, const uint32_t* witness, const uint32_t* rs, uint32_t* proof_out, int flags,
               double* t_ms);

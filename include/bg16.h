@@ -104,8 +104,16 @@ int bg_qap_h(const uint32_t* a, const uint32_t* b, const uint32_t* c, int log2n,
  * (prove.bend:64-116) and the final combination (groth16.bend:298-308). */
 typedef struct bg_pk bg_pk;
 
-/* Loads a FORMAT.md key directory (header.bin, r1cs_{a,b,c}.bin, pk_*.bin) into HBM. */
+/* Loads a FORMAT.md key directory (header.bin, r1cs_{a,b,c}.bin, pk_*.bin) into HBM.
+ * Every file is validated on the host (bend2/tools/verify_bin.py:24-64): sizes, limb
+ * words below 2^16, CSR offsets and columns, coefficients below r, and every G1 / G2
+ * point (pk_g1, pk_g2 and the five queries) below q and on its curve (or all-zero
+ * infinity); a failing file gives BG_ERR_IO. */
 int bg_pk_load(const char* dir, bg_pk** out);
+/* Frees a loaded key.  A key must outlive every proof that uses it: while a bg_prove
+ * call on it runs or an engine request of it is queued or running (until the proof is
+ * done, i.e. until bg_engine_wait can return it), this returns BG_ERR_ARG and the key
+ * stays loaded. */
 int bg_pk_free(bg_pk* pk);
 /* header: num_constraints, num_instance, num_witness, domain_log2, n_a, n_b, n_h, n_l */
 int bg_pk_header(const bg_pk* pk, uint32_t* header8);
@@ -115,7 +123,9 @@ int bg_pk_device_bytes(const bg_pk* pk, uint64_t* bytes);
 
 /* One proof.  witness: num_instance + num_witness Fr (Montgomery, FORMAT.md witness.bin
  * order), host memory in either encoding, or with BG_MEM_DEVICE a packed device buffer
- * already resident in HBM; rs: r then s (Montgomery, FORMAT.md rs.bin, host).
+ * already resident in HBM; rs: r then s (Montgomery, FORMAT.md rs.bin, host).  A host
+ * witness element or r / s at or above r gives BG_ERR_ARG (a device witness is the
+ * caller's to keep reduced).
  * proof_out (host): A (G1), B (G2), C (G1) affine Montgomery (the proof_ref.bin layout),
  * in the flags' encoding.
  * t_ms (optional, 9 doubles): device time of upload, evals, qap, msm_a, msm_b1, msm_b2,

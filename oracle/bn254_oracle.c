@@ -765,6 +765,12 @@ static void eval_rows(u64 *ev, const u32 *off, const u32 *col, const u64 *coef, 
     }
 }
 
+/* one matrix's evaluation rows: ev[i] = sum_e coef[e] z[col[e]] for rows i < nc (Montgomery) */
+void or_eval_rows(u64 *ev, const u32 *off, const u32 *col, const u64 *coef, u32 nc, const u64 *z) {
+    or_init_once();
+    eval_rows(ev, off, col, coef, nc, z);
+}
+
 int or_prove(const u32 *hdr,
              const u64 *z,
              const u32 *a_off, const u32 *a_col, const u64 *a_coef,

@@ -317,6 +317,7 @@ def lib():
             "or_threads": [], "or_set_threads": [ctypes.c_int],
             "or_g1_gen_mul_batch": [P, P, U64], "or_g2_gen_mul_batch": [P, P, U64],
             "or_fr_dot": [P, P, P, U64], "or_poly_eval": [P, P, U64, P],
+            "or_eval_rows": [P, P, P, P, ctypes.c_uint32, P],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -467,6 +468,17 @@ def rand_scalars(rng: np.random.Generator, n: int, bound: int = P_R) -> np.ndarr
             while from_words(out[i]) >= bound:
                 out[i] = rng.integers(0, 2**64, size=4, dtype=np.uint64)
                 out[i, 3] = rng.integers(0, top + 1, dtype=np.uint64)
+    return out
+
+
+def eval_rows(off: np.ndarray, col: np.ndarray, coef: np.ndarray, z: np.ndarray, nc: int, d: int) -> np.ndarray:
+    """(d, 4) u64 Montgomery: row i < nc = sum of coef * z over the CSR row, zero beyond."""
+    out = np.zeros((d, 4), dtype=np.uint64)
+    off = np.ascontiguousarray(off, dtype=np.uint32)
+    col = np.ascontiguousarray(col, dtype=np.uint32)
+    coef = np.ascontiguousarray(coef, dtype=np.uint64)
+    z = np.ascontiguousarray(z, dtype=np.uint64)
+    lib().or_eval_rows(_ptr(out), _ptr(off), _ptr(col), _ptr(coef), nc, _ptr(z))
     return out
 
 

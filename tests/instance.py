@@ -84,9 +84,24 @@ def eval_vectors(inst: dict) -> tuple[list[int], list[int], list[int]]:
     return pad(a), pad(b), pad(c)
 
 
-def _poly_at(vals: list[int], lg: int, t: int) -> int:
-    """value at t of the polynomial interpolating vals on the size-2^lg domain"""
-    arr = O.ints_to_array(vals, R)
+def eval_vectors_mont(inst: dict) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """eval_vectors as (d, 4) u64 Montgomery arrays through the oracle's C rows (the
+    same values; test_oracle_math pins the two against each other)."""
+    nc, ni, nw, lg = (int(x) for x in inst["header"][:4])
+    d = 1 << lg
+    out = []
+    for m in "abc":
+        v = O.eval_rows(inst[f"{m}_off"], inst[f"{m}_col"], inst[f"{m}_coef"], inst["z"], nc, d)
+        if m == "a":
+            v[nc:nc + ni] = inst["z"][:ni]
+        out.append(v)
+    return out[0], out[1], out[2]
+
+
+def _poly_at(vals, lg: int, t: int) -> int:
+    """value at t of the polynomial interpolating vals (ints, or a (d, 4) Montgomery
+    array) on the size-2^lg domain"""
+    arr = vals if isinstance(vals, np.ndarray) else O.ints_to_array(vals, R)
     coeffs = O.ntt(arr, lg, 1)  # ifft
     return O.unmont(O.poly_eval(coeffs, O.mont(t, R)), R)
 
@@ -105,8 +120,8 @@ def closed_form(inst: dict):
     t, al, be, ga, de = td["tau"], td["alpha"], td["beta"], td["gamma"], td["delta"]
     r = O.unmont(O.from_words(inst["r"]), R)
     s = O.unmont(O.from_words(inst["s"]), R)
-    z = [O.unmont(O.from_words(v), R) for v in inst["z"]]
-    ea, eb, ec = eval_vectors(inst)
+    z = [O.unmont(O.from_words(v), R) for v in inst["z"][:ni]]
+    ea, eb, ec = eval_vectors_mont(inst)
     At, Bt, Ct = (_poly_at(v, lg, t) for v in (ea, eb, ec))
     # u_i, v_i, w_i at t for the instance variables (their columns and input rows)
     u = [0] * ni
@@ -114,11 +129,10 @@ def closed_form(inst: dict):
     w = [0] * ni
     for m, acc in (("a", u), ("b", v), ("c", w)):
         off, col = inst[f"{m}_off"], inst[f"{m}_col"]
-        coef = [O.unmont(O.from_words(c), R) for c in inst[f"{m}_coef"]]
-        for row in range(nc):
-            for e in range(int(off[row]), int(off[row + 1])):
-                if int(col[e]) < ni:
-                    acc[int(col[e])] = (acc[int(col[e])] + coef[e] * lagrange_at(row, lg, t)) % R
+        rows = np.repeat(np.arange(nc), np.diff(off.astype(np.int64)))
+        for e in np.nonzero(col < ni)[0]:
+            cf = O.unmont(O.from_words(inst[f"{m}_coef"][e]), R)
+            acc[int(col[e])] = (acc[int(col[e])] + cf * lagrange_at(int(rows[e]), lg, t)) % R
     for i in range(ni):
         u[i] = (u[i] + lagrange_at(nc + i, lg, t)) % R
     ic = sum(z[i] * (be * u[i] + al * v[i] + w[i]) for i in range(ni)) % R

@@ -140,3 +140,49 @@ def test_engine_batches_two_keys_of_one_shape(loaded, keys, slots):
         want = solo[k] if fmt == "words16" else bendg.pack16(solo[k].reshape(-1, 16)).reshape(-1)
         assert np.array_equal(proof, want), f"key {k} {fmt} ticket {t}"
     eng.close()
+
+
+def test_key_free_refused_while_queued(loaded, keys):
+    """bg_pk_free refuses a key whose proof is still queued in an engine (the key stays
+    loaded and the proof completes unchanged); once the proof is collected it frees."""
+    _, solo = loaded
+    pk = bendg.ProvingKey(keys[4])
+    eng = bendg.Engine(slots=1)
+    pks, _ = loaded
+    z12, rs12 = _files(keys[12])
+    ahead = [eng.submit(pks[12], z12, rs12) for _ in range(8)]  # keep the one slot busy
+    z, rs = _files(keys[4])
+    t = eng.submit(pk, z, rs)
+    with pytest.raises(bendg.BgError) as ei:
+        pk.close()
+    assert ei.value.code == -1
+    for a in ahead:
+        eng.wait(a)
+    proof, _ = eng.wait(t)
+    assert np.array_equal(proof, solo[4])
+    pk.close()  # no proof of the key left: frees
+    eng.close()
+
+
+def test_unreduced_inputs_rejected(loaded, keys):
+    """A witness element or r / s at or above r is an argument error (FORMAT.md stores
+    reduced Montgomery values), from bg_prove and from bg_engine_submit alike."""
+    pks, solo = loaded
+    z, rs = _files(keys[8])
+    r_mod = 21888242871839275222246405745257275088548364400416034343698204186575808495617
+    limbs = np.array([(r_mod >> (16 * i)) & 0xFFFF for i in range(16)], dtype=np.uint32)
+    zb = z.copy()
+    zb[16 * 3:16 * 4] = limbs
+    rb = rs.copy()
+    rb[16:32] = limbs
+    eng = bendg.Engine(slots=1)
+    for w, r in ((zb, rs), (z, rb)):
+        with pytest.raises(bendg.BgError) as ei:
+            pks[8].prove(w, r, fmt="words16")
+        assert ei.value.code == -1
+        with pytest.raises(bendg.BgError) as ei:
+            eng.submit(pks[8], w, r)
+        assert ei.value.code == -1
+    proof, _ = pks[8].prove(z, rs, fmt="words16")  # the key still proves
+    assert np.array_equal(proof, solo[8])
+    eng.close()

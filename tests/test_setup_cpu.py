@@ -118,3 +118,18 @@ def test_fixed_base_and_synthetic_bases(group):
     assert np.array_equal(pts, bendg.fixed_base_mul(group, k))
     assert np.array_equal(pts, bendg.gen_bases(group, 5, 64))
     assert not np.array_equal(pts, bendg.gen_bases(group, 6, 64))
+
+
+def test_c_eval_rows_equal_python_rows(key4, tmp_path):
+    """The closed form's evaluation vectors come from the oracle's C rows; they equal the
+    big-integer rows (CSR x z, plus the instance rows of A) at K=4 and K=7."""
+    from instance import eval_vectors, eval_vectors_mont
+    from oracle import oracle as O
+    d7 = tmp_path / "k7"
+    bendg.setup_square_chain(d7, 7, seed=5)
+    for d in (key4, d7):
+        inst = load(d)
+        py = eval_vectors(inst)
+        cm = eval_vectors_mont(inst)
+        for a, b in zip(py, cm):
+            assert a == O.array_to_ints(b, O.P_R)
