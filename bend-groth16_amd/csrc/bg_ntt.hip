@@ -193,33 +193,48 @@ static bgh::Fr omega_of(int log2, bool inverse) {
   return w;
 }
 
-// out[i] = c * x^i for i < n, built from 2^10-entry low / high tables on the device
-static void build_geom(DevArray& out, uint64_t n, bgh::Fr x, bgh::Fr c, bool canon, hipStream_t s) {
-  const int lb = 10;
-  uint64_t nlo = std::min<uint64_t>(n, 1ull << lb);
-  uint64_t nhi = (n + (1ull << lb) - 1) >> lb;
-  std::vector<uint32_t> lo(8 * nlo), hi(8 * nhi);
+// Host halves of the geometric table c * x^i: lo[i] = c x^i (i < nlo), hi[j] = x^(2^10 j)
+// (j < nhi), as packed words.
+static constexpr int GEOM_LO_BITS = 10;
+static void geom_host(uint32_t* lo, uint64_t nlo, uint32_t* hi, uint64_t nhi, bgh::Fr x, bgh::Fr c) {
   bgh::Fr v = c;
   for (uint64_t i = 0; i < nlo; i++) {
     v.to_u32(&lo[8 * i]);
     v = v * x;
   }
   bgh::Fr step = bgh::Fr::one();
-  for (int i = 0; i < (1 << lb); i++) step = step * x;  // x^1024
+  for (int i = 0; i < (1 << GEOM_LO_BITS); i++) step = step * x;  // x^1024
   bgh::Fr h = bgh::Fr::one();
   for (uint64_t i = 0; i < nhi; i++) {
     h.to_u32(&hi[8 * i]);
     h = h * step;
   }
+}
+
+// out[i] = c * x^i for i < n on the device, from the host halves staged through the
+// device buffers dlo / dhi (2^10 and n / 2^10 elements); blocks until done, so the host
+// halves may be reused at once.
+static void geom_fill(uint32_t* out, uint64_t n, bgh::Fr x, bgh::Fr c, bool canon, uint32_t* hlo, uint32_t* hhi,
+                      uint32_t* dlo, uint32_t* dhi, hipStream_t s) {
+  const uint64_t nlo = std::min<uint64_t>(n, 1ull << GEOM_LO_BITS);
+  const uint64_t nhi = (n + (1ull << GEOM_LO_BITS) - 1) >> GEOM_LO_BITS;
+  geom_host(hlo, nlo, hhi, nhi, x, c);
+  BG_HIP(hipMemcpyAsync(dlo, hlo, nlo * 32, hipMemcpyHostToDevice, s));
+  BG_HIP(hipMemcpyAsync(dhi, hhi, nhi * 32, hipMemcpyHostToDevice, s));
+  k_geom<<<nblk(n, 256), 256, 0, s>>>(out, n, dlo, dhi, GEOM_LO_BITS, canon ? 1 : 0);
+  BG_HIP(hipGetLastError());
+  BG_HIP(hipStreamSynchronize(s));
+}
+
+static void build_geom(DevArray& out, uint64_t n, bgh::Fr x, bgh::Fr c, bool canon, hipStream_t s) {
+  const uint64_t nlo = std::min<uint64_t>(n, 1ull << GEOM_LO_BITS);
+  const uint64_t nhi = (n + (1ull << GEOM_LO_BITS) - 1) >> GEOM_LO_BITS;
+  std::vector<uint32_t> lo(8 * nlo), hi(8 * nhi);
   DevArray dlo, dhi;
   dlo.alloc(lo.size() * 4);
   dhi.alloc(hi.size() * 4);
-  BG_HIP(hipMemcpyAsync(dlo.ptr, lo.data(), lo.size() * 4, hipMemcpyHostToDevice, s));
-  BG_HIP(hipMemcpyAsync(dhi.ptr, hi.data(), hi.size() * 4, hipMemcpyHostToDevice, s));
   out.alloc(n * 32);
-  k_geom<<<nblk(n, 256), 256, 0, s>>>(out.as<uint32_t>(), n, dlo.as<uint32_t>(), dhi.as<uint32_t>(), lb, canon ? 1 : 0);
-  BG_HIP(hipGetLastError());
-  BG_HIP(hipStreamSynchronize(s));  // host tables go out of scope
+  geom_fill(out.as<uint32_t>(), n, x, c, canon, lo.data(), hi.data(), dlo.as<uint32_t>(), dhi.as<uint32_t>(), s);
 }
 
 NttDomain& ntt_domain(int log2) {
@@ -272,26 +287,31 @@ static Fp to_fp(const uint32_t* w) {
 }
 
 // Pass list of a 2^log2 transform in DIF order (DIT runs it backwards): strided passes
-// of at most NTT_TILE_LOG2 - 1 stages, then one contiguous pass of NTT_TILE_LOG2 stages.
+// of at most NTT_TILE_LOG2 - 1 stages, then one contiguous pass of NTT_TILE_LOG2 stages
+// (at most 4 passes up to 2^28).
 struct NttPass {
   int k0, t, logC;
 };
-static std::vector<NttPass> ntt_plan(int log2) {
-  std::vector<NttPass> out;
+struct NttPlan {
+  NttPass p[4];
+  int count = 0;
+};
+static NttPlan ntt_plan(int log2) {
+  NttPlan out;
   if (log2 <= NTT_TILE_LOG2) {
-    out.push_back({0, log2, 0});
+    out.p[out.count++] = {0, log2, 0};
     return out;
   }
   const int rem = log2 - NTT_TILE_LOG2;
   const int np = (rem + NTT_TILE_LOG2 - 2) / (NTT_TILE_LOG2 - 1);
   int k0 = 0;
-  for (int p = 0; p < np; p++) {
-    int t = rem / np + (p < rem % np ? 1 : 0);
+  for (int q = 0; q < np; q++) {
+    int t = rem / np + (q < rem % np ? 1 : 0);
     int S_log = log2 - k0 - t;
-    out.push_back({k0, t, std::min(NTT_TILE_LOG2 - t, S_log)});
+    out.p[out.count++] = {k0, t, std::min(NTT_TILE_LOG2 - t, S_log)};
     k0 += t;
   }
-  out.push_back({k0, NTT_TILE_LOG2, 0});
+  out.p[out.count++] = {k0, NTT_TILE_LOG2, 0};
   return out;
 }
 
@@ -309,18 +329,18 @@ static void launch_pass(bool dit, const uint32_t* src, uint32_t* dst, int log2, 
 // natural order in -> bit-reversed order out; scale multiplies the (natural) input
 static void dif(const uint32_t* src, uint32_t* dst, int log2, const uint32_t* tw, const uint32_t* scale,
                 hipStream_t s, unsigned batch = 1) {
-  auto plan = ntt_plan(log2);
-  for (size_t i = 0; i < plan.size(); i++)
-    launch_pass(false, i == 0 ? src : dst, dst, log2, plan[i], tw, i == 0 ? scale : nullptr, s, batch);
+  const NttPlan plan = ntt_plan(log2);
+  for (int i = 0; i < plan.count; i++)
+    launch_pass(false, i == 0 ? src : dst, dst, log2, plan.p[i], tw, i == 0 ? scale : nullptr, s, batch);
 }
 
 // bit-reversed order in -> natural order out; scale multiplies the (bit-reversed) input
 static void dit(const uint32_t* src, uint32_t* dst, int log2, const uint32_t* tw, const uint32_t* scale,
                 hipStream_t s, unsigned batch = 1) {
-  auto plan = ntt_plan(log2);
-  for (size_t i = plan.size(); i-- > 0;)
-    launch_pass(true, i + 1 == plan.size() ? src : dst, dst, log2, plan[i], tw,
-                i + 1 == plan.size() ? scale : nullptr, s, batch);
+  const NttPlan plan = ntt_plan(log2);
+  for (int i = plan.count; i-- > 0;)
+    launch_pass(true, i + 1 == plan.count ? src : dst, dst, log2, plan.p[i], tw, i + 1 == plan.count ? scale : nullptr,
+                s, batch);
 }
 
 static void gather(const uint32_t* in, uint32_t* out, int log2, const uint32_t* scale, const uint32_t* c,
@@ -344,27 +364,117 @@ void ntt_run(uint32_t* d, uint32_t* tmp, int log2, int direction, hipStream_t s)
     gather(tmp, d, log2, nullptr, nullptr, s);
 }
 
+// The tables qap_h reads, wherever they live.
+struct QapTables {
+  const uint32_t* tw_fwd;
+  const uint32_t* tw_inv;
+  const uint32_t* ninv_g_rev;  // n^-1 g^rev(j)
+  const uint32_t* ninv_gi;     // n^-1 g^-j (Montgomery out) or its canonical-out form
+  uint32_t zinv[8];
+};
+
+static void qap_h_core(uint32_t* a, uint32_t* b, uint32_t* c, uint32_t* h, int log2, const QapTables& T,
+                       hipStream_t s, unsigned batch) {
+  const uint64_t n = 1ull << log2;
+  uint32_t* v3[3] = {a, b, c};
+  for (uint32_t* v : v3) {
+    // coefficients (bit-reversed, times n) -> n^-1 g^k scale on load -> coset evaluations
+    dif(v, v, log2, T.tw_inv, nullptr, s, batch);
+    dit(v, v, log2, T.tw_fwd, T.ninv_g_rev, s, batch);
+  }
+  k_qap_point<<<nblk(n * batch, 256), 256, 0, s>>>(a, b, c, n * batch, to_fp(T.zinv));
+  BG_HIP(hipGetLastError());
+  dif(a, a, log2, T.tw_inv, nullptr, s, batch);
+  gather(a, h, log2, T.ninv_gi, nullptr, s, batch);
+}
+
 void qap_h_run(uint32_t* a, uint32_t* b, uint32_t* c, uint32_t* h, uint32_t* tmp, int log2, bool canonical_out,
                hipStream_t s, unsigned batch) {
   (void)tmp;
   NttDomain& D = ntt_domain(log2);
-  uint64_t n = D.n;
-  uint32_t* v3[3] = {a, b, c};
-  for (uint32_t* v : v3) {
-    // coefficients (bit-reversed, times n) -> n^-1 g^k scale on load -> coset evaluations
-    dif(v, v, log2, D.tw_inv.as<uint32_t>(), nullptr, s, batch);
-    dit(v, v, log2, D.tw_fwd.as<uint32_t>(), D.sc_ninv_g_rev.as<uint32_t>(), s, batch);
-  }
-  k_qap_point<<<nblk(n * batch, 256), 256, 0, s>>>(a, b, c, n * batch, to_fp(D.zinv));
-  BG_HIP(hipGetLastError());
-  dif(a, a, log2, D.tw_inv.as<uint32_t>(), nullptr, s, batch);
-  gather(a, h, log2, canonical_out ? D.sc_ninv_gi_c.as<uint32_t>() : D.sc_ninv_gi.as<uint32_t>(), nullptr, s,
-         batch);
+  QapTables T{D.tw_fwd.as<uint32_t>(), D.tw_inv.as<uint32_t>(), D.sc_ninv_g_rev.as<uint32_t>(),
+              canonical_out ? D.sc_ninv_gi_c.as<uint32_t>() : D.sc_ninv_gi.as<uint32_t>(), {}};
+  memcpy(T.zinv, D.zinv, 32);
+  qap_h_core(a, b, c, h, log2, T, s, batch);
 }
+
+// ------------------------------------------------------------------ allocation-free qap_h
+// bg_qap_h_noalloc keeps everything beyond its caller's buffers in one file-scope device
+// array: copies of a, b, c (transformed in place), the domain's tables and the staging
+// of the tables' host halves.  Sized for domains up to 2^NOALLOC_QAP_MAX_LOG2; the tables
+// of the last domain stay in the array and are rebuilt when a call asks for another size.
+static constexpr int NOALLOC_QAP_MAX_LOG2 = 17;
+static constexpr uint64_t NA_N = 1ull << NOALLOC_QAP_MAX_LOG2;
+static constexpr uint64_t NA_LO = 8ull << GEOM_LO_BITS;            // words of the low half
+static constexpr uint64_t NA_HI = 8 * (NA_N >> GEOM_LO_BITS);      // words of the high half
+// a, b, c, tw_fwd, tw_inv, n^-1 g^rev(j), n^-1 g^-j (8 words each per element), the
+// n / 2 flat powers of omega, the staged halves
+static constexpr uint64_t NA_WORDS = 7 * 8 * NA_N + 4 * NA_N + NA_LO + NA_HI;
+static __device__ __align__(256) uint32_t scratch[NA_WORDS];
+
+static constexpr int NA_MAX_DEVICES = 64;
+static uint32_t* g_na_ptr[NA_MAX_DEVICES];   // scratch's device address, per device
+static int g_na_log2[NA_MAX_DEVICES];        // domain whose tables scratch holds (0: none)
+static uint32_t g_na_hlo[NA_LO], g_na_hhi[NA_HI];  // host halves (reused per table)
 
 }  // namespace bg
 
 using namespace bg;
+
+extern "C" int bg_qap_h_noalloc(const uint32_t* a, const uint32_t* b, const uint32_t* cc, int log2n, uint32_t* h) {
+  return guard([&] {
+    BG_REQUIRE(a && b && cc && h, BG_ERR_ARG, "bg_qap_h_noalloc: null argument");
+    BG_REQUIRE(log2n >= 1 && log2n <= NOALLOC_QAP_MAX_LOG2, BG_ERR_ARG,
+               "bg_qap_h_noalloc: log2n %d out of range 1..%d (the static scratch holds 2^%d-element domains)",
+               log2n, NOALLOC_QAP_MAX_LOG2, NOALLOC_QAP_MAX_LOG2);
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> lk(c.mu);
+    BG_REQUIRE(c.device >= 0 && c.device < NA_MAX_DEVICES, BG_ERR_ARG, "bg_qap_h_noalloc: device %d", c.device);
+    uint32_t*& base = g_na_ptr[c.device];
+    if (!base) {
+      // the array's address on this device (each device holds its own copy)
+      void* p = nullptr;
+      BG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+      base = static_cast<uint32_t*>(p);
+    }
+    const uint64_t n = 1ull << log2n;
+    uint32_t* da = base;
+    uint32_t* db = da + 8 * NA_N;
+    uint32_t* dc = db + 8 * NA_N;
+    uint32_t* tw_fwd = dc + 8 * NA_N;
+    uint32_t* tw_inv = tw_fwd + 8 * NA_N;
+    uint32_t* g_rev = tw_inv + 8 * NA_N;
+    uint32_t* ninv_gi = g_rev + 8 * NA_N;
+    uint32_t* flat = ninv_gi + 8 * NA_N;  // 4 NA_N words
+    uint32_t* dlo = flat + 4 * NA_N;
+    uint32_t* dhi = dlo + NA_LO;
+    hipStream_t s = c.stream;
+    const bgh::Fr g = bgh::Fr::to_mont(bgh::Fr::raw(5)), gi = g.inv();
+    const bgh::Fr ninv = bgh::Fr::to_mont(bgh::Fr::raw(n)).inv();
+    if (g_na_log2[c.device] != log2n) {
+      g_na_log2[c.device] = 0;  // tables incomplete until the end of this block
+      for (int dir = 0; dir < 2; dir++) {
+        geom_fill(flat, n / 2, omega_of(log2n, dir != 0), bgh::Fr::one(), false, g_na_hlo, g_na_hhi, dlo, dhi, s);
+        k_stage_twiddles<<<nblk(n, 256), 256, 0, s>>>(flat, dir ? tw_inv : tw_fwd, log2n);
+        BG_HIP(hipGetLastError());
+      }
+      // n^-1 g^j staged in the a copy, then put in bit-reversed order
+      geom_fill(da, n, g, ninv, false, g_na_hlo, g_na_hhi, dlo, dhi, s);
+      k_gather<<<nblk(n, 256), 256, 0, s>>>(da, g_rev, log2n, nullptr, Fp{}, 0);
+      BG_HIP(hipGetLastError());
+      geom_fill(ninv_gi, n, gi, ninv, false, g_na_hlo, g_na_hhi, dlo, dhi, s);
+      g_na_log2[c.device] = log2n;
+    }
+    QapTables T{tw_fwd, tw_inv, g_rev, ninv_gi, {}};
+    uint64_t e4[4] = {n, 0, 0, 0};
+    (g.pow(e4) - bgh::Fr::one()).inv().to_u32(T.zinv);
+    BG_HIP(hipMemcpyAsync(da, a, 32 * n, hipMemcpyDeviceToDevice, s));
+    BG_HIP(hipMemcpyAsync(db, b, 32 * n, hipMemcpyDeviceToDevice, s));
+    BG_HIP(hipMemcpyAsync(dc, cc, 32 * n, hipMemcpyDeviceToDevice, s));
+    qap_h_core(da, db, dc, h, log2n, T, s, 1);
+    BG_HIP(hipStreamSynchronize(s));
+  });
+}
 
 extern "C" int bg_ntt(uint32_t* data, int log2n, int direction, int flags) {
   return guard([&] {

@@ -98,6 +98,17 @@ int bg_ntt(uint32_t* data, int log2n, int direction, int flags);
 int bg_qap_h(const uint32_t* a, const uint32_t* b, const uint32_t* c, int log2n, uint32_t* h,
              int flags);
 
+/* qap_h without allocation: no counterpart in the reference.  For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  a, b, c, h: packed Montgomery device buffers of 2^log2n
+ * elements (a, b, c read-only); h receives what bg_qap_h returns for the same inputs.
+ * All working memory -- copies of a, b, c and the domain's tables -- lives in one static
+ * device array of about 30 MiB (domains up to 2^17: log2n 1..17, larger gives
+ * BG_ERR_ARG), held by each device the library runs on until the process ends.  The
+ * call makes no hipMalloc / hipFree and no other allocation; it blocks until h is
+ * written.  Calls share the array: calls that overlap on one device race. */
+int bg_qap_h_noalloc(const uint32_t* a, const uint32_t* b, const uint32_t* c, int log2n, uint32_t* h);
+
 /* ------------------------------------------------------------------ resident-key prover
  * Reproduces bend2/prove.bend's Pr.run (:129-162) on the device: evaluation vectors
  * (groth16.bend:162-208), qap_h, the five MSMs in the reference's order and slots

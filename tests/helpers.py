@@ -46,3 +46,38 @@ def g2_points(rnd: random.Random, n: int, scalars=None):
         O.lib().or_g2_mul_aff(O._ptr(r), O._ptr(gen), O._ptr(kk))
         out[i] = r
     return out.view(np.uint32).reshape(n, 32), ks
+
+
+class DeviceBuffers:
+    """Device memory for tests that call the C ABI with device pointers (HIP runtime
+    through ctypes); every buffer is freed by free_all()."""
+
+    def __init__(self):
+        import ctypes
+        self.ct = ctypes
+        self.L = ctypes.CDLL("libamdhip64.so")
+        self.L.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.L.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        self.L.hipFree.argtypes = [ctypes.c_void_p]
+        self.ptrs = []
+
+    def alloc(self, nbytes: int) -> int:
+        p = self.ct.c_void_p()
+        assert self.L.hipMalloc(self.ct.byref(p), max(nbytes, 4)) == 0
+        self.ptrs.append(p.value)
+        return p.value
+
+    def upload(self, arr: np.ndarray) -> int:
+        a = np.ascontiguousarray(arr)
+        p = self.alloc(a.nbytes)
+        assert self.L.hipMemcpy(self.ct.c_void_p(p), a.ctypes.data_as(self.ct.c_void_p), a.nbytes, 1) == 0
+        return p
+
+    def download(self, p: int, arr: np.ndarray) -> np.ndarray:
+        assert self.L.hipMemcpy(arr.ctypes.data_as(self.ct.c_void_p), self.ct.c_void_p(p), arr.nbytes, 2) == 0
+        return arr
+
+    def free_all(self) -> None:
+        for p in self.ptrs:
+            self.L.hipFree(self.ct.c_void_p(p))
+        self.ptrs = []
