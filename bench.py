@@ -100,7 +100,10 @@ def ensure_key(k: int, seed: int, base: Path) -> Path:
 # product: 64 a*b, 64 m*p and 8 reduction digits of the 32-bit-digit Montgomery product (the
 # work unit; the kernels' 29-bit-digit product issues 171 multiplies for the same product)
 MULS_PER_FQ = 136
-FQ_PER_MADD = {1: 10, 2: 28}   # madd-2008-s: 8M + 2S (G2: Karatsuba 3 / complex square 2)
+# Fq products per mixed addition in the reference's formula (madd-2007-bl, 7M + 4S; G2 with
+# Karatsuba Fq2 products: 29), the algorithmic work unit of a sorted (bucket, point) entry;
+# the kernel's own XYZZ mixed addition (madd-2008-s) takes 10 (G1) / 28 (G2)
+FQ_PER_MADD = {1: 11, 2: 29}
 
 
 R_MOD = 21888242871839275222246405745257275088548364400416034343698204186575808495617
@@ -154,15 +157,18 @@ def probe_peak() -> float:
 
 
 def traffic_from_profile() -> dict | None:
-    """HBM bytes per launch of the G1 accumulation kernel, from the committed rocprofv3
-    --pmc summary (FETCH_SIZE + WRITE_SIZE scaled by a device copy of known size)."""
+    """HBM bytes per launch of the G1 accumulation kernel and its VALU issue fraction, from
+    the committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE scaled by a device copy
+    of known size; SQ_INSTS_VALU over the launch's duration)."""
     f = ROOT / "profiles" / "msm_g1_accumulate_2^19_traffic.md"
     if not f.exists():
         return None
+    out = {"source": f"profiles/{f.name}"}
     for line in f.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
-            return {"bytes": float(line.split(":")[1]), "source": f"profiles/{f.name}"}
-    return None
+        key, _, val = line.partition(":")
+        if key in ("traffic_bytes_per_launch", "valu_issue_frac_published_clock", "valu_issue_frac_measured_clock"):
+            out[key] = float(val)
+    return out if "traffic_bytes_per_launch" in out else None
 
 
 # MI355X's published VALU issue rate: 256 CUs x 64 lanes x 2.4 GHz peak engine clock, one
@@ -173,7 +179,8 @@ SPEC_MUL_RATE = 256 * 64 * 2.4e9
 def roofline(standalone: dict, peak_mul: float) -> dict:
     """Roofline of the G1 accumulation kernel (the proof's dominant kernel) from its
     standalone average launch duration (HIP events on the library stream): one mixed add
-    (10 Fq products x 136 multiplies) per sorted (bucket, point) entry.  `peak` is the
+    (11 Fq products x 136 multiplies, the reference's madd-2007-bl) per sorted (bucket,
+    point) entry.  `peak` is the
     published issue rate; `peak_measured` is the v_mad_u64_u32 probe on this machine."""
     k = standalone["msm_g1_accumulate"]
     entries = bendg.msm_plan_info(k["n"], 1)["entries"]
@@ -184,7 +191,9 @@ def roofline(standalone: dict, peak_mul: float) -> dict:
     return {"bound": "valu", "kernel": "k_accumulate<G1> (msm_g1_accumulate)", "achieved": round(achieved, 3),
             "peak": round(spec, 3), "unit": "T int32-mul/s", "frac": round(achieved / spec, 4),
             "peak_measured": round(peak_mul / 1e12, 3), "frac_of_measured": round(achieved / (peak_mul / 1e12), 4),
-            "traffic": tr["bytes"] if tr else None, "traffic_source": tr["source"] if tr else None,
+            "traffic": tr["traffic_bytes_per_launch"] if tr else None, "traffic_source": tr["source"] if tr else None,
+            "valu_issue_frac": tr.get("valu_issue_frac_published_clock") if tr else None,
+            "valu_issue_frac_measured_clock": tr.get("valu_issue_frac_measured_clock") if tr else None,
             "algorithmic_bytes": entries * (8 + 64), "work_per_launch": work, "points_per_launch": k["n"],
             "entries_per_launch": entries, "avg_launch_ms": k["avg_ms"],
             "peak_note": "published VALU issue rate 256 CU x 64 lanes x 2.4 GHz; peak_measured: "
@@ -578,8 +587,9 @@ def primitives_mode(a) -> dict:
 
 def roofline_mode(a) -> dict:
     """The default line's standalone pass alone (profiled by rocprofv3 for the committed
-    roofline summary), plus a 256 MiB device copy whose counters calibrate FETCH_SIZE /
-    WRITE_SIZE."""
+    roofline and counter summaries), plus a 256 MiB device copy whose counters calibrate
+    FETCH_SIZE / WRITE_SIZE, then five 2^20 forward NTTs and five qap_h of the proof's
+    domain."""
     key = ensure_key(a.k, a.seed, Path(a.keys))
     bendg.init(0)
     pk = bendg.ProvingKey(key)
@@ -595,9 +605,31 @@ def roofline_mode(a) -> dict:
     hip.free(dst)
     peak = probe_peak()
     alone = standalone_msm_pass(key, hdr, 5)
+    # the transforms alone: a 2^20 forward NTT and the proof's qap_h at 2^19 (device
+    # buffers, uniform Montgomery inputs), five launches each
+    rng = np.random.default_rng(11)
+    L = bendg.lib()
+    flags = bendg.FMT_PACKED | bendg.MEM_DEVICE
+    d = hip.upload(uniform_fr(rng, 1 << 20))
+    lg = hdr[3]
+    abc = [hip.upload(uniform_fr(rng, 1 << lg)) for _ in range(3)]
+    dh = hip.alloc(32 << lg)
+    # one untimed call of each first: the domains' tables are built on first use
+    bendg._check(L.bg_ntt(ctypes.c_void_p(d), 20, bendg.NTT_FFT, flags))
+    bendg._check(L.bg_qap_h(*(ctypes.c_void_p(x) for x in abc), lg, ctypes.c_void_p(dh), flags))
+    bendg.timing_reset()
+    bendg.set_timing(True)
+    for _ in range(5):
+        bendg._check(L.bg_ntt(ctypes.c_void_p(d), 20, bendg.NTT_FFT, flags))
+        bendg._check(L.bg_qap_h(*(ctypes.c_void_p(x) for x in abc), lg, ctypes.c_void_p(dh), flags))
+    bendg.set_timing(False)
+    transforms = {k: round(v[0], 4) for k, v in bendg.timing_report().items()}
+    for x in [d, dh] + abc:
+        hip.free(x)
     return {"metric": "roofline of the G1 accumulation kernel", "value": roofline(alone, peak)["frac"],
             "unit": "fraction of the published VALU issue rate", "roofline": roofline(alone, peak),
-            "kernel_ms_standalone": {k: v["avg_ms"] for k, v in alone.items()}, "copy_bytes": nb}
+            "kernel_ms_standalone": {k: v["avg_ms"] for k, v in alone.items()}, "copy_bytes": nb,
+            "transform_ms": transforms}
 
 
 def main() -> int:
