@@ -28,6 +28,10 @@ Other modes (one JSON line each):
   --mode primitives  G1 MSM 2^20 and 2^22, G2 MSM 2^20 (bases k_i G, uniform scalars,
                      checked against the closed form) and Fr NTT 2^20 / 2^24 forward and
                      inverse (round trip checked).
+  --mode tune        sweeps window width, slice target and reduction shape of the MSMs and
+                     tile and radix of the NTT on the device and writes the parameter table
+                     the planners read (tune_gfx950.txt).
+  --mode roofline    the standalone MSM pass and the transforms alone (profiled by rocprofv3).
 """
 from __future__ import annotations
 
@@ -632,6 +636,164 @@ def roofline_mode(a) -> dict:
             "transform_ms": transforms}
 
 
+def tune_mode(a) -> dict:
+    """Device-measured parameter table (replaces Msm.pick_c / pick_fd, bend2/src/msm.bend:37-44,
+    and Ntt.fd, bend2/src/ntt.bend:382-385).  Each candidate runs with a one-entry table (plus
+    the entries already chosen for the same proof) loaded by bg_tune_load, timed on device
+    buffers; the best of each sweep is kept:
+      - NTT tile (2^9 / 2^10 / 2^11 elements) x radix (2 / 4) per log2 n: qap_h for 2^11..2^21
+        (the proofs' domains), the forward transform for 2^22..2^24;
+      - windowed MSMs (arbitrary bases) per log2 n, G1 2^10..2^22 and G2 2^10..2^20: window
+        width c around the cost model's, then the accumulation slice target L, the level-1
+        group m and the level-2 block K;
+      - fixed-base MSMs of SquareChain proofs 2^10..2^20: c of the G1 batch (A, B1, L), of H
+        and of B2, one after the other, by whole-proof time (the key is reloaded per candidate:
+        its tables are built for the width chosen at load).
+    Writes the table to <keys>/tune_gfx950.txt (copied next to the library to take effect)
+    and prints it in the JSON line."""
+    import math
+    bendg.init(0)
+    hip = Hip()
+    L = bendg.lib()
+    flags = bendg.FMT_PACKED | bendg.MEM_DEVICE
+    base = Path(a.keys)
+    base.mkdir(parents=True, exist_ok=True)
+    cand = base / "tune_candidate.txt"
+    reps = 5
+
+    def load(lines):
+        cand.write_text("\n".join(lines) + "\n")
+        bendg.tune_load(str(cand))
+
+    def timed(fn, reps=reps):
+        fn()
+        hip.sync()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            hip.sync()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)) * 1e3
+
+    def keep_model(res, model_key, margin=0.015):
+        """The fastest candidate, unless it beats the model's choice by less than the margin
+        (measurement noise): then the model's choice."""
+        best = min(res, key=res.get)
+        return best if res[best] < (1 - margin) * res[model_key] else model_key
+
+    table, results = [], {"ntt": {}, "msm": {}, "fixed": {}, "validation": {}}
+    t_start = time.perf_counter()
+    rng = np.random.default_rng(a.seed)
+    # ---- NTT tile and radix
+    for lg in range(11, 25):
+        n = 1 << lg
+        if lg <= 21:
+            bufs = [hip.upload(uniform_fr(rng, n)) for _ in range(3)]
+            dh = hip.alloc(32 * n)
+            fn = lambda: bendg._check(L.bg_qap_h(*(ctypes.c_void_p(x) for x in bufs), lg, ctypes.c_void_p(dh), flags))  # noqa: E731
+        else:
+            bufs = [hip.upload(uniform_fr(rng, n))]
+            dh = None
+            fn = lambda: bendg._check(L.bg_ntt(ctypes.c_void_p(bufs[0]), lg, bendg.NTT_FFT, flags))  # noqa: E731
+        res = {}
+        for tile in (9, 10, 11):
+            for radix in (2, 4):
+                load([f"ntt {lg} tile={tile} radix={radix}"])
+                res[(tile, radix)] = timed(fn)
+        best = keep_model(res, (10, 4))
+        table.append(f"ntt {lg} tile={best[0]} radix={best[1]}")
+        results["ntt"][lg] = {f"{t}/{r}": round(v, 4) for (t, r), v in res.items()}
+        log(f"ntt 2^{lg}: {table[-1]} ({res[best]:.3f} ms; default {res[(10, 4)]:.3f} ms)")
+        for x in bufs + ([dh] if dh else []):
+            hip.free(x)
+    ntt_lines = list(table)
+    # ---- windowed MSMs
+    for group, top in ((1, 22), (2, 20)):
+        pts = bendg.gen_bases(group, 99 + group, 1 << top)
+        dp = hip.upload(pts)
+        ds = hip.upload(uniform_fr(rng, 1 << top))
+        dout = hip.alloc(128)
+        f = L.bg_msm_g1 if group == 1 else L.bg_msm_g2
+        for lg in range(10, top + 1):
+            n = 1 << lg
+            fn = lambda: bendg._check(f(ctypes.c_void_p(dp), ctypes.c_void_p(ds), n, ctypes.c_void_p(dout), flags, 0))  # noqa: E731
+            load([])
+            c0 = bendg.msm_plan_info(n, group)["c"]
+            key = f"msm {group} windowed {lg}"
+            res = {}
+            for c in range(max(4, c0 - 3), min(20, c0 + 2) + 1):
+                load([f"{key} c={c}"])
+                res[f"c={c}"] = timed(fn)
+            best = keep_model(res, f"c={c0}")
+            sweeps = [("L", (20, 30, 40, 60)), ("m", (2, 4, 8)),
+                      ("K", (64, 128, 256) if group == 1 else (32, 64, 128))]
+            for name, vals in sweeps:
+                base_entry = best
+                for v in vals:
+                    entry = f"{base_entry} {name}={v}"
+                    load([f"{key} {entry}"])
+                    res[entry] = timed(fn)
+                best = keep_model({k: v for k, v in res.items() if k == base_entry or k.startswith(base_entry + " ")},
+                                  base_entry)
+            table.append(f"{key} {best}")
+            results["msm"][f"g{group}_{lg}"] = {k: round(v, 4) for k, v in res.items()}
+            log(f"msm G{group} 2^{lg}: {best} ({res[best]:.3f} ms; model c={c0} {res[f'c={c0}']:.3f} ms)")
+        for x in (dp, ds, dout):
+            hip.free(x)
+    # ---- fixed-base MSMs inside proofs
+    for k in range(10, 21):
+        kdir = ensure_key(k, 42 + k, base)
+        z, rs = witness_files(kdir)
+        dz = hip.upload(z)
+        hdr = [int(x) for x in np.fromfile(kdir / "header.bin", dtype=np.uint32)]
+        roles = [("g1", 1, hdr[4] + hdr[5] + hdr[7]), ("h", 1, hdr[6]), ("b2", 2, hdr[5])]
+        load(ntt_lines)
+        pk = bendg.ProvingKey(kdir)
+        c_model = pk.windows()
+        pk.close()
+        chosen = {}
+        res_k = {}
+        for role, group, total in roles:
+            half = round(2 * math.log2(total)) / 2
+            key = f"msm {group} fixed {half:g}"
+            res = {}
+            c0 = c_model[role]
+            for c in range(max(8, c0 - 2), min(20, c0 + 2) + 1):
+                load(ntt_lines + list(chosen.values()) + [f"{key} c={c}"])
+                pk = bendg.ProvingKey(kdir)
+                assert pk.windows()[role] == c, (role, c, pk.windows())
+                res[c] = timed(lambda: pk.prove_device(dz, rs), reps=9)
+                pk.close()
+            bc = keep_model(res, c0)
+            chosen[role] = f"{key} c={bc}"
+            res_k[role] = {str(c): round(v, 4) for c, v in res.items()}
+            log(f"proof 2^{k} {role}: c={bc} ({res[bc]:.3f} ms; model c={c0} {res[c0]:.3f} ms)")
+        table += list(chosen.values())
+        results["fixed"][k] = res_k
+        # validation: the proof under the cost model alone vs under the tuned entries,
+        # alternated three times
+        val = {"model": [], "tuned": []}
+        for _ in range(3):
+            for name, lines in (("model", []), ("tuned", ntt_lines + list(chosen.values()))):
+                load(lines)
+                pk = bendg.ProvingKey(kdir)
+                val[name].append(timed(lambda: pk.prove_device(dz, rs), reps=9))
+                pk.close()
+        results["validation"][k] = {n: round(float(np.median(v)), 4) for n, v in val.items()}
+        log(f"proof 2^{k}: model {results['validation'][k]['model']:.3f} ms, tuned "
+            f"{results['validation'][k]['tuned']:.3f} ms")
+        hip.free(dz)
+    text = "\n".join(["# Device-tuned parameters for gfx950 (MI355X), written by bench.py --mode tune.",
+                      "# msm <group> <windowed|fixed> <log2 total points, half steps> c= L= m= K=;  ntt <log2 n> tile= radix="]
+                     + table) + "\n"
+    (base / "tune_gfx950.txt").write_text(text)
+    cand.unlink()
+    bendg.tune_load(None)
+    return {"metric": "device-tuned parameter table", "value": len(table), "unit": "entries", "n_gpus": 1,
+            "table": text, "results": results, "tune_s": round(time.perf_counter() - t_start, 1)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -642,7 +804,7 @@ def main() -> int:
     ap.add_argument("--keys", default=os.environ.get("BG16_KEYS", "/tmp/bg16_keys"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mix", action="store_true", help="single mode: skip the 64-proof batched mix")
-    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve", "primitives", "roofline"])
+    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve", "primitives", "roofline", "tune"])
     ap.add_argument("--slots", type=lambda v: [int(x) for x in v.split(",")], default=[4, 8],
                     help="engine slots (batch: comma list to sweep)")
     ap.add_argument("--rates", type=lambda v: [float(x) for x in v.split(",")], default=[100.0, 200.0, 300.0],
@@ -662,7 +824,7 @@ def main() -> int:
         os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, max(1, hwq)))
     if a.mode != "single":
         line = {"batch": batch_mode, "serve": serve_mode, "primitives": primitives_mode,
-                "roofline": roofline_mode}[a.mode](a)
+                "roofline": roofline_mode, "tune": tune_mode}[a.mode](a)
         print(json.dumps(line), flush=True)
         return 0
 

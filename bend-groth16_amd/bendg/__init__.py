@@ -86,6 +86,10 @@ def lib() -> ctypes.CDLL:
         "bg_gen_bases": [I, U64, U64, P, P, I],
         "bg_verify": [P, U64, P, P, ctypes.c_uint32, I],
         "bg_pairing_check": [P, P, ctypes.c_uint32, I],
+        "bg_tune_load": [ctypes.c_char_p],
+        "bg_pk_windows": [P, P],
+        "bg_tune_lookup": [I, I, I, U64, P],
+        "bg_qap_h_noalloc": [P, P, P, I, P],
     }
     for name, args in sigs.items():
         fn = getattr(L, name, None)
@@ -302,6 +306,24 @@ def msm_plan_info(n: int, group: int = 1, window_bits: int = 0) -> dict:
     _check(lib().bg_msm_plan_info(n, group, window_bits, _p(info)))
     keys = ("c", "windows", "buckets", "entries", "L", "acc_threads", "groups1", "outputs2")
     return {k: int(v) for k, v in zip(keys, info)}
+
+
+def tune_load(path: str | None = None) -> None:
+    """(Re)load the device-tuned parameter table (None: the default next to the library)."""
+    _check(lib().bg_tune_load(path.encode() if path else None))
+
+
+def tune_lookup(kind: str, size: int, group: int = 1, fixed: bool = False) -> dict | None:
+    """The table's entry: kind "msm" (size = total points) -> {c, L, m, K}; kind "ntt"
+    (size = log2 n) -> {tile, radix}; None when the table has no entry."""
+    v = np.zeros(4, dtype=np.int32)
+    rc = lib().bg_tune_lookup(0 if kind == "msm" else 1, group, 1 if fixed else 0, size, _p(v))
+    if rc == 1:
+        return None
+    _check(rc)
+    if kind == "msm":
+        return dict(zip(("c", "L", "m", "K"), (int(x) for x in v)))
+    return {"tile": int(v[0]), "radix": int(v[1])}
 
 
 # ---------------------------------------------------------------- timing

@@ -29,6 +29,12 @@ class ProvingKey:
     def handle(self):
         return self._h
 
+    def windows(self) -> dict:
+        """Window widths of the resident fixed-base tables: G1 batch (A, B1, L), H, B2."""
+        c = np.zeros(3, dtype=np.int32)
+        _check(lib().bg_pk_windows(self._h, _p(c)))
+        return {"g1": int(c[0]), "h": int(c[1]), "b2": int(c[2])}
+
     def device_bytes(self) -> int:
         b = np.zeros(1, dtype=np.uint64)
         _check(lib().bg_pk_device_bytes(self._h, _p(b)))

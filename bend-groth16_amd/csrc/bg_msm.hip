@@ -2,6 +2,7 @@
 #include "bg_msm.hpp"
 #include "bg_curve.hpp"
 #include "bg_host_ec.hpp"
+#include "bg_tune.hpp"
 #include <algorithm>
 #include <map>
 #include <memory>
@@ -47,7 +48,11 @@ static void finish_plan(MsmPlan& p, int group) {
   // set of 2^16 or more buckets takes groups of 4: in pairs its level 2 would hold 512
   // waves of G2 registers (a SIMD's file each) for ~0.65 ms while the other MSMs of a
   // proof accumulate (2^18 proof 5.22 -> 5.12 ms; below 2^16 buckets pairs stay faster)
+  // the device-tuned table's entry for this batch, if any (environment overrides win)
+  MsmTune tt;
+  const bool tuned = tune_msm(group, p.fixed, p.n, tt);
   uint32_t mm = (uint32_t)tune_env(group == 1 ? "BG16_MSM_M" : "BG16_MSM_M_G2");
+  if (mm == 0 && tuned) mm = (uint32_t)tt.m;
   if (mm == 0) mm = group == 1 || p.W > 1 ? 8 : (p.B >= 65536 ? 4 : 2);
   mm = mm >= 8 ? 8 : (mm >= 4 ? 4 : 2);
   p.m = std::min<uint32_t>(mm, p.B);
@@ -62,6 +67,7 @@ static void finish_plan(MsmPlan& p, int group) {
   // override: rounded down to a power of two (the level-2 scan and trees need one, and
   // K must divide the group count)
   uint32_t kk = (uint32_t)std::max(0, tune_env("BG16_MSM_K"));
+  if (!kk && tuned) kk = (uint32_t)tt.K;
   if (kk) {
     uint32_t k2 = 1;
     while (k2 * 2 <= kk) k2 *= 2;
@@ -81,7 +87,8 @@ static void finish_plan(MsmPlan& p, int group) {
     L = lenv;
   } else {
     const uint64_t cap = accumulate_threads_per_round(group);
-    const uint64_t k = std::max<uint64_t>(1, p.entries / (40 * cap));
+    const uint64_t target = tuned && tt.L ? (uint64_t)tt.L : 40;
+    const uint64_t k = std::max<uint64_t>(1, p.entries / (target * cap));
     L = (p.entries + k * cap - 1) / (k * cap);
     // G2 MSMs of less than a device round: the rule above gives 2-3 entries a slice, and
     // the merge tree then adds the runs of a bucket in several dependent rounds of G2
@@ -119,6 +126,8 @@ MsmPlan msm_plan_batch(const uint64_t* ns, uint32_t M, int group, int window_bit
   for (uint32_t i = 0; i < M; i++) n += ns[i];
   p.n = n;
   int c = window_bits;
+  MsmTune tt;
+  if (c <= 0 && tune_msm(group, false, n, tt) && tt.c) c = tt.c;
   if (c <= 0) {
     // every window has its own buckets
     double best = 1e300;
@@ -140,6 +149,10 @@ MsmPlan msm_plan_batch(const uint64_t* ns, uint32_t M, int group, int window_bit
 int fixed_window_bits(const uint64_t* ns, uint32_t M, int group) {
   int env = tune_env(group == 1 ? "BG16_FIXED_C_G1" : "BG16_FIXED_C_G2");
   if (env >= 2 && env <= 22) return env;  // other values: the planner's width
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < M; i++) total += ns[i];
+  MsmTune tt;
+  if (tune_msm(group, true, total, tt) && tt.c) return tt.c;
   // every window shares one set of buckets
   double best = 1e300;
   int c = 16;

@@ -1,6 +1,6 @@
 // Radix-2 NTT over BN254 Fr on gfx950 (C ABI: bg_ntt, bg_qap_h).  See bg_ntt.hpp.
 //
-// A transform is a short list of passes; each pass runs up to NTT_TILE_LOG2 radix-2
+// A transform is a short list of passes; each pass runs up to tile (2^9..2^11 elements) radix-2
 // stages of one workgroup's elements in LDS (one global read and write per pass).
 // Decimation in frequency takes natural order to bit-reversed order and decimation in
 // time takes bit-reversed order back to natural order, so qap_h chains inverse (DIF) and
@@ -11,12 +11,15 @@
 #include "bg_msm.hpp"
 #include "bg_field.hpp"
 #include "bg_host_ec.hpp"
+#include "bg_tune.hpp"
 #include <map>
 #include <memory>
 
 namespace bg {
 
-static constexpr int NTT_TILE_LOG2 = 10;   // elements per workgroup: 1024 (32 KB of LDS)
+// elements per workgroup: 2^TILE (default 1024, 32 KB of LDS; the tuning table may pick 512
+// or 2048 per size), stages two at a time in registers (radix 4) or one at a time
+static constexpr int NTT_TILE_DEFAULT = 10;
 static constexpr int NTT_THREADS = 256;
 
 __global__ void k_geom(uint32_t* __restrict__ out, uint64_t n, const uint32_t* __restrict__ lo,
@@ -72,11 +75,11 @@ __global__ void k_gather(const uint32_t* __restrict__ in, uint32_t* __restrict__
 // of the per-stage table tw.  scale (optional) multiplies each element by scale[index] as
 // it is loaded.
 // src and dst may alias (in-place pass).
-template <bool DIT>
+template <bool DIT, int TILE, bool R4>
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass(const uint32_t* src, uint32_t* dst, int log2, int k0, int t,
                                                           int logC, const uint32_t* __restrict__ tw,
                                                           const uint32_t* __restrict__ scale) {
-  __shared__ Fp sh[1 << NTT_TILE_LOG2];
+  __shared__ Fp sh[1 << TILE];
   src += 8 * ((uint64_t)blockIdx.y << log2);
   dst += 8 * ((uint64_t)blockIdx.y << log2);
   const int S_log = log2 - k0 - t;
@@ -104,7 +107,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass(const uint32_t* src, u
   // row distances, and applies both stages in registers: half the LDS round trips and
   // barriers of one stage at a time, the same four twiddle products.
   int s = 0;
-  for (; s + 1 < t; s += 2) {
+  for (; R4 && s + 1 < t; s += 2) {
     const int ka = DIT ? k0 + t - 1 - s : k0 + s;  // applied first
     const int kb = DIT ? ka - 1 : ka + 1;          // applied second
     const int ls = log2 - 1 - (DIT ? ka : kb) - S_log;
@@ -140,8 +143,8 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass(const uint32_t* src, u
     }
     __syncthreads();
   }
-  if (s < t) {
-    // an odd stage count leaves one radix-2 stage
+  for (; s < t; s++) {
+    // radix-2 stages: the one an odd stage count leaves, or all of them without radix 4
     const int k = DIT ? k0 + t - 1 - s : k0 + s;
     const int lmr = log2 - 1 - k - S_log;  // local row distance m / S = 2^lmr
     const uint32_t nbf = cnt >> 1;
@@ -287,42 +290,63 @@ static Fp to_fp(const uint32_t* w) {
 }
 
 // Pass list of a 2^log2 transform in DIF order (DIT runs it backwards): strided passes
-// of at most NTT_TILE_LOG2 - 1 stages, then one contiguous pass of NTT_TILE_LOG2 stages
-// (at most 4 passes up to 2^28).
+// of at most tile - 1 stages, then one contiguous pass of tile stages (at most 4 passes up
+// to 2^28 for tiles of 2^9 and more).
 struct NttPass {
   int k0, t, logC;
 };
 struct NttPlan {
   NttPass p[4];
   int count = 0;
+  int tile = NTT_TILE_DEFAULT;
+  bool r4 = true;
 };
 static NttPlan ntt_plan(int log2) {
   NttPlan out;
-  if (log2 <= NTT_TILE_LOG2) {
+  NttTune tt;
+  if (tune_ntt(log2, tt)) {
+    if (tt.tile_log2) out.tile = tt.tile_log2;
+    if (tt.radix) out.r4 = tt.radix == 4;
+  }
+  const int T = out.tile;
+  if (log2 <= T) {
     out.p[out.count++] = {0, log2, 0};
     return out;
   }
-  const int rem = log2 - NTT_TILE_LOG2;
-  const int np = (rem + NTT_TILE_LOG2 - 2) / (NTT_TILE_LOG2 - 1);
+  const int rem = log2 - T;
+  const int np = (rem + T - 2) / (T - 1);
   int k0 = 0;
   for (int q = 0; q < np; q++) {
     int t = rem / np + (q < rem % np ? 1 : 0);
     int S_log = log2 - k0 - t;
-    out.p[out.count++] = {k0, t, std::min(NTT_TILE_LOG2 - t, S_log)};
+    out.p[out.count++] = {k0, t, std::min(T - t, S_log)};
     k0 += t;
   }
-  out.p[out.count++] = {k0, NTT_TILE_LOG2, 0};
+  out.p[out.count++] = {k0, T, 0};
   return out;
 }
 
-// batch: number of contiguous 2^log2-element transforms (grid y)
-static void launch_pass(bool dit, const uint32_t* src, uint32_t* dst, int log2, const NttPass& p,
-                        const uint32_t* tw, const uint32_t* scale, hipStream_t s, unsigned batch) {
-  const dim3 wgs(1u << (log2 - p.t - p.logC), batch);
-  if (dit)
-    k_ntt_pass<true><<<wgs, NTT_THREADS, 0, s>>>(src, dst, log2, p.k0, p.t, p.logC, tw, scale);
+template <bool DIT, int TILE>
+static void launch_pass_t(bool r4, dim3 wgs, const uint32_t* src, uint32_t* dst, int log2, const NttPass& p,
+                          const uint32_t* tw, const uint32_t* scale, hipStream_t s) {
+  if (r4)
+    k_ntt_pass<DIT, TILE, true><<<wgs, NTT_THREADS, 0, s>>>(src, dst, log2, p.k0, p.t, p.logC, tw, scale);
   else
-    k_ntt_pass<false><<<wgs, NTT_THREADS, 0, s>>>(src, dst, log2, p.k0, p.t, p.logC, tw, scale);
+    k_ntt_pass<DIT, TILE, false><<<wgs, NTT_THREADS, 0, s>>>(src, dst, log2, p.k0, p.t, p.logC, tw, scale);
+}
+
+// batch: number of contiguous 2^log2-element transforms (grid y)
+static void launch_pass(bool dit, const uint32_t* src, uint32_t* dst, int log2, const NttPlan& plan,
+                        const NttPass& p, const uint32_t* tw, const uint32_t* scale, hipStream_t s, unsigned batch) {
+  const dim3 wgs(1u << (log2 - p.t - p.logC), batch);
+  switch (plan.tile * 2 + (dit ? 1 : 0)) {
+    case 18: launch_pass_t<false, 9>(plan.r4, wgs, src, dst, log2, p, tw, scale, s); break;
+    case 19: launch_pass_t<true, 9>(plan.r4, wgs, src, dst, log2, p, tw, scale, s); break;
+    case 22: launch_pass_t<false, 11>(plan.r4, wgs, src, dst, log2, p, tw, scale, s); break;
+    case 23: launch_pass_t<true, 11>(plan.r4, wgs, src, dst, log2, p, tw, scale, s); break;
+    case 21: launch_pass_t<true, 10>(plan.r4, wgs, src, dst, log2, p, tw, scale, s); break;
+    default: launch_pass_t<false, 10>(plan.r4, wgs, src, dst, log2, p, tw, scale, s); break;
+  }
   BG_HIP(hipGetLastError());
 }
 
@@ -331,7 +355,7 @@ static void dif(const uint32_t* src, uint32_t* dst, int log2, const uint32_t* tw
                 hipStream_t s, unsigned batch = 1) {
   const NttPlan plan = ntt_plan(log2);
   for (int i = 0; i < plan.count; i++)
-    launch_pass(false, i == 0 ? src : dst, dst, log2, plan.p[i], tw, i == 0 ? scale : nullptr, s, batch);
+    launch_pass(false, i == 0 ? src : dst, dst, log2, plan, plan.p[i], tw, i == 0 ? scale : nullptr, s, batch);
 }
 
 // bit-reversed order in -> natural order out; scale multiplies the (bit-reversed) input
@@ -339,8 +363,8 @@ static void dit(const uint32_t* src, uint32_t* dst, int log2, const uint32_t* tw
                 hipStream_t s, unsigned batch = 1) {
   const NttPlan plan = ntt_plan(log2);
   for (int i = plan.count; i-- > 0;)
-    launch_pass(true, i + 1 == plan.count ? src : dst, dst, log2, plan.p[i], tw, i + 1 == plan.count ? scale : nullptr,
-                s, batch);
+    launch_pass(true, i + 1 == plan.count ? src : dst, dst, log2, plan, plan.p[i], tw,
+                i + 1 == plan.count ? scale : nullptr, s, batch);
 }
 
 static void gather(const uint32_t* in, uint32_t* out, int log2, const uint32_t* scale, const uint32_t* c,

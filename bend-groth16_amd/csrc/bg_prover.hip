@@ -337,6 +337,15 @@ extern "C" int bg_pk_header(const bg_pk* pk, uint32_t* header8) {
   });
 }
 
+extern "C" int bg_pk_windows(const bg_pk* pk, int* c3) {
+  return guard([&] {
+    BG_REQUIRE(pk && c3, BG_ERR_ARG, "bg_pk_windows: null argument");
+    c3[0] = pk->fixed ? pk->c_g1 : 0;
+    c3[1] = pk->fixed ? pk->c_h : 0;
+    c3[2] = pk->fixed ? pk->c_b2 : 0;
+  });
+}
+
 extern "C" int bg_pk_device_bytes(const bg_pk* pk, uint64_t* bytes) {
   return guard([&] {
     BG_REQUIRE(pk && bytes, BG_ERR_ARG, "bg_pk_device_bytes: null argument");

@@ -86,6 +86,20 @@ int bg_msm_batch(int group, uint32_t count, const uint32_t* const* points, const
  * the algorithmic work of a launch. */
 int bg_msm_plan_info(uint64_t n, int group, int window_bits, uint64_t* info8);
 
+/* ------------------------------------------------------------------ tuned parameters
+ * Replaces the reference's fixed heuristics Msm.pick_c / Msm.pick_fd
+ * (bend2/src/msm.bend:37-44) and Ntt.fd (bend2/src/ntt.bend:382-385): a table measured on
+ * the device (`bench.py --mode tune`) of MSM window width, accumulation slice target and
+ * reduction shape per (group, windowed / fixed-base, log2 points in half steps), and of NTT tile
+ * and radix per log2 n.  The planners read it first and fall back to the cost model.
+ * bg_tune_load(path): (re)loads the table (NULL: $BG16_TUNE_FILE, else tune_gfx950.txt
+ * next to the library; a missing file is an empty table, a malformed one BG_ERR_IO).
+ * bg_tune_lookup(kind, group, fixed, size, values4): kind 0 = MSM (size = total points;
+ * values4 = c, L, m, K), kind 1 = NTT (size = log2 n; values4 = tile log2, radix); 0 = an
+ * entry exists, 1 = none (values4 zero), negative = error.  0 in a value: planner's choice. */
+int bg_tune_load(const char* path);
+int bg_tune_lookup(int kind, int group, int fixed, uint64_t size, int* values4);
+
 /* ------------------------------------------------------------------ NTT
  * Replaces Ntt.fft / ifft / coset_fft / coset_ifft (bend2/src/ntt.bend:431-442), in
  * place on 2^log2n Montgomery Fr elements, natural order in and out.  ifft scales by
@@ -128,6 +142,9 @@ int bg_pk_load(const char* dir, bg_pk** out);
 int bg_pk_free(bg_pk* pk);
 /* header: num_constraints, num_instance, num_witness, domain_log2, n_a, n_b, n_h, n_l */
 int bg_pk_header(const bg_pk* pk, uint32_t* header8);
+/* Window widths of the key's fixed-base tables, chosen at load (tuned table, else cost
+ * model): c3 = G1 batch (A, B1, L), H, B2; zeros when the key keeps plain arrays. */
+int bg_pk_windows(const bg_pk* pk, int* c3);
 /* Device bytes held by a loaded key (the resident queries, matrices and NTT tables;
  * per-proof work arrays live in the prover slots). */
 int bg_pk_device_bytes(const bg_pk* pk, uint64_t* bytes);

@@ -72,3 +72,28 @@ def test_qap_h_vs_oracle(bg, lg):
     want = O.qap_h(a, b, c, lg)
     assert np.array_equal(got, want)
     assert not want[-1].any()  # h[d-1] = 0 for a satisfied instance
+
+
+@pytest.mark.parametrize("tile", [9, 10, 11])
+@pytest.mark.parametrize("radix", [2, 4])
+def test_ntt_tile_radix_variants(bg, tmp_path, tile, radix):
+    """Every pass variant the tuning table can select (tile 2^9 / 2^10 / 2^11 elements,
+    radix 2 / 4) gives the same transforms and qap_h, at a one-pass and a multi-pass size."""
+    t = tmp_path / "tune.txt"
+    sizes = (8, 12, 19)
+    t.write_text("".join(f"ntt {lg} tile={tile} radix={radix}\n" for lg in sizes))
+    bendg.tune_load(str(t))
+    try:
+        for lg in sizes:
+            rng = np.random.default_rng(1000 + 10 * tile + radix + lg)
+            a = O.rand_scalars(rng, 1 << lg)
+            print(f"ntt tile={tile} radix={radix} 2^{lg}", flush=True)
+            for direction in (0, 3):
+                got = bendg.ntt(_u32(a), lg, direction).view(np.uint64)
+                assert np.array_equal(got, O.ntt(a, lg, direction)), (lg, direction)
+            if lg <= 12:
+                b, c = O.rand_scalars(rng, 1 << lg), O.rand_scalars(rng, 1 << lg)
+                h = bendg.qap_h(_u32(a), _u32(b), _u32(c), lg).view(np.uint64)
+                assert np.array_equal(h, O.qap_h(a, b, c, lg))
+    finally:
+        bendg.tune_load(None)

@@ -127,6 +127,38 @@ def test_prove_windowed_and_fixed_base_agree(bg, keys, monkeypatch):
     pk_plain.close()
 
 
+def test_prove_tuned_widths_used_and_exact(bg, keys, tmp_path):
+    """A key loaded under a tuning table takes the table's fixed-base widths for its G1
+    batch, H and B2 (the planner reads the table first), and its proof is unchanged and
+    equal to the closed form."""
+    d = keys[10]
+    z, rs = _files(d)
+    pk0 = bendg.ProvingKey(d)
+    ref, _ = pk0.prove(z, rs)
+    w0 = pk0.windows()
+    pk0.close()
+    hdr = [int(x) for x in np.fromfile(d / "header.bin", dtype=np.uint32)]
+    import math
+    half = lambda n: round(2 * math.log2(n)) / 2  # noqa: E731
+    want = {"g1": 9 if w0["g1"] != 9 else 10, "h": 11 if w0["h"] != 11 else 12, "b2": 8 if w0["b2"] != 8 else 9}
+    t = tmp_path / "tune.txt"
+    t.write_text(f"msm 1 fixed {half(hdr[4] + hdr[5] + hdr[7]):g} c={want['g1']}\n"
+                 f"msm 1 fixed {half(hdr[6]):g} c={want['h']}\n"
+                 f"msm 2 fixed {half(hdr[5]):g} c={want['b2']}\n")
+    bendg.tune_load(str(t))
+    try:
+        pk = bendg.ProvingKey(d)
+        assert pk.windows() == want
+        got, _ = pk.prove(z, rs)
+        pk.close()
+    finally:
+        bendg.tune_load(None)
+    assert np.array_equal(got, ref)
+    eA, eB, eC = points_of(closed_form(load(d)))
+    A, B, C = _split(bendg.pack16(got.reshape(-1, 16)).reshape(-1))
+    assert np.array_equal(A, eA) and np.array_equal(B, eB) and np.array_equal(C, eC)
+
+
 @pytest.mark.parametrize("m", ["2", "4", "8"])
 def test_prove_g2_group_widths_agree(bg, keys, monkeypatch, m):
     """The G2 bucket reduction's level-1 group width (pairs below 2^16 fixed-base buckets,
