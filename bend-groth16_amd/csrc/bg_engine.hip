@@ -10,26 +10,22 @@
 // Proofs of different sizes run concurrently on different slots: one proof's
 // latency-bound reductions and host finishing overlap the others' accumulation.  Keys stay
 // resident (bg_pk_load) and are shared read-only by all slots.  Every proof is computed
-// exactly as bg_prove computes it alone.
+// exactly as bg_prove computes it alone.  The queue, admission and batching logic is the
+// device-free Scheduler of bg_sched.hpp; this file supplies its prover-facing parts.
 //
 // The reference proves one instance per process run (bend2/prove.bend:129-162); this is
 // the serving layer the batched and served configurations run on.
 #include "bg_prover.hpp"
+#include "bg_sched.hpp"
 #include <chrono>
-#include <condition_variable>
 #include <algorithm>
-#include <deque>
-#include <map>
 #include <memory>
 #include <string.h>
-#include <thread>
 #include <vector>
-
-struct bg_engine;
 
 namespace bg {
 
-using Clock = std::chrono::steady_clock;
+using Clock = sched::Clock;
 
 struct Request {
   uint64_t ticket = 0;
@@ -45,30 +41,13 @@ struct Request {
   std::string err;
 };
 
-}  // namespace bg
-
-struct bg_engine {
-  std::mutex mu;
-  std::condition_variable cv_work, cv_done;
-  std::deque<bg::Request*> queue;
-  std::map<uint64_t, std::unique_ptr<bg::Request>> all;
-  std::vector<std::thread> workers;
-  std::vector<uint64_t> reserved;  // per slot: device bytes its work arrays may hold
-  uint64_t budget = 0, next_ticket = 1, completed = 0;
-  int running = 0;  // slots proving now
-  bool stop = false;
-  int device = 0;
-};
-
-namespace bg {
-
 // Proofs of one shape a slot takes together: 16 up to domain 2^14, 4 up to 2^16, 2 up to
 // 2^18, and none from 2^19, where one proof fills the device.  Smaller batches as proofs
 // grow keep several slots' batches in flight (a batch completes all at once): bursts of 32
 // same-key proofs on 4 slots measured best at these sizes (2^15: 4 -> 1,175 proofs/s,
 // 16 -> 798; 2^17: 2 -> 685, 8 -> 405; 2^19: 1 -> 212, 2 -> 200).  BG16_ENGINE_BATCH caps
 // it (1 disables batching).
-static size_t batch_limit(bg_pk* pk) {
+static size_t batch_limit_of(bg_pk* pk) {
   uint32_t hdr[8];
   if (bg_pk_header(pk, hdr) != 0) return 1;
   const uint32_t lg = hdr[3];  // log2 of the domain: constraints + 2 rounded up
@@ -78,128 +57,57 @@ static size_t batch_limit(bg_pk* pk) {
   return lim;
 }
 
-// Longest a partial batch waits for more requests of its shape while the device is busy
-// (BG16_ENGINE_WAIT_US, default 2000 us).
-static int64_t batch_wait_us() {
-  static const int64_t w = [] {
-    const char* e = getenv("BG16_ENGINE_WAIT_US");
-    return e ? (int64_t)atoll(e) : (int64_t)2000;
-  }();
-  return w;
-}
-
-static uint64_t total_reserved(const bg_engine* e) {
-  uint64_t t = 0;
-  for (uint64_t r : e->reserved) t += r;
-  return t;
-}
-
-static void worker_main(bg_engine* e, int slot_id) {
-  std::unique_ptr<ProveSlot> slot;
-  try {
-    slot = std::make_unique<ProveSlot>();
-  } catch (const Error&) {
+// The scheduler's device-facing parts: slots are prover slots, a group of one request is
+// proved by prove_on, a larger group (keys of one shape) by prove_batch_on.
+struct EngineOps {
+  using Request = bg::Request;
+  using Slot = ProveSlot;
+  std::unique_ptr<Slot> make_slot() {
+    try {
+      return std::make_unique<ProveSlot>();
+    } catch (const Error&) {
+      return nullptr;
+    }
   }
-  for (;;) {
-    std::vector<Request*> group;
-    {
-      std::unique_lock<std::mutex> lk(e->mu);
-      for (;;) {
-        if (e->stop) return;
-        if (!e->queue.empty()) {
-          Request* head = e->queue.front();
-          // the head and the next queued requests of the same key shape
-          std::vector<Request*> cand = {head};
-          const size_t lim = batch_limit(head->pk);
-          for (auto it = e->queue.begin() + 1; it != e->queue.end() && cand.size() < lim; ++it)
-            if (prove_batchable(head->pk, (*it)->pk)) cand.push_back(*it);
-          // a partial batch waits for more requests while another slot keeps the device busy,
-          // up to max_wait after the head arrived (a full batch, or an idle device, goes now)
-          const auto max_wait = std::chrono::microseconds(batch_wait_us());
-          const auto ready_at = head->t_submit + max_wait;
-          if (cand.size() < lim && e->running > 0 && Clock::now() < ready_at) {
-            e->cv_work.wait_until(lk, ready_at);
-            continue;
-          }
-          uint64_t mine = e->reserved[slot_id];
-          uint64_t others = total_reserved(e) - mine;
-          uint64_t need = head->need * cand.size();
-          if (cand.size() > 1 && others + std::max(mine, need) > e->budget) {
-            cand.resize(1);  // the batch does not fit: the head alone
-            need = head->need;
-          }
-          uint64_t grown = std::max(mine, need);
-          const bool fits = others + grown <= e->budget;
-          const bool fits_after_release = !fits && mine && others + need <= e->budget;
-          if (fits || fits_after_release) {
-            // claim the requests before the lock may be dropped
-            for (Request* r : cand) e->queue.erase(std::find(e->queue.begin(), e->queue.end(), r));
-            group = cand;
-            e->running++;
-            e->reserved[slot_id] = fits ? grown : need;
-            if (fits_after_release) {
-              // the batch fits once this idle slot gives its arrays back
-              lk.unlock();
-              if (slot) slot->release_buffers();
-              lk.lock();
-            }
-            break;
-          }
-          if (mine) {
-            // idle and holding memory that a waiting request needs: give it back
-            e->reserved[slot_id] = 0;
-            lk.unlock();
-            if (slot) slot->release_buffers();
-            lk.lock();
-            e->cv_work.notify_all();
-            continue;
-          }
-        }
-        e->cv_work.wait(lk);
-      }
-    }
-    const auto t_start = Clock::now();
-    int rc = 0;
-    std::string msg;
-    if (!slot) {
-      rc = BG_ERR_DEVICE;
-      msg = "engine worker could not create its streams";
-    } else {
-      rc = guard([&] {
-        if (group.size() == 1) {
-          Request* r = group[0];
-          prove_on(r->pk, *slot, r->dev_witness ? r->dev_witness : r->witness.data(), r->rs, r->proof, r->flags,
-                   nullptr);
-        } else {
-          std::vector<ProveJob> jobs;
-          for (Request* r : group)
-            jobs.push_back({r->pk, r->dev_witness ? r->dev_witness : r->witness.data(), r->rs, r->proof, r->flags});
-          prove_batch_on(jobs.data(), (int)jobs.size(), *slot);
-        }
-      });
-      if (rc < 0) msg = bg_last_error();
-    }
-    const auto t_end = Clock::now();
-    {
-      std::lock_guard<std::mutex> lk(e->mu);
-      e->running--;
-      for (Request* r : group) {
-        r->t_start = t_start;
-        r->t_end = t_end;
-        r->status = rc < 0 ? rc : 0;
-        r->err = msg;
-        e->completed++;
-        pk_release(r->pk);
-      }
-      // actual footprint after the run (grow-only buffers)
-      if (slot) e->reserved[slot_id] = std::max(e->reserved[slot_id], slot->device_bytes());
-    }
-    e->cv_done.notify_all();
-    e->cv_work.notify_all();
+  size_t batch_limit(const Request& r) { return batch_limit_of(r.pk); }
+  bool batchable(const Request& head, const Request& other) { return prove_batchable(head.pk, other.pk); }
+  // Longest a partial batch waits for more requests of its shape while the device is busy
+  // (BG16_ENGINE_WAIT_US, default 2000 us).
+  int64_t batch_wait_us() {
+    static const int64_t w = [] {
+      const char* e = getenv("BG16_ENGINE_WAIT_US");
+      return e ? (int64_t)atoll(e) : (int64_t)2000;
+    }();
+    return w;
   }
-}
+  int run(Slot& slot, std::vector<Request*>& group, std::string& msg) {
+    const int rc = guard([&] {
+      if (group.size() == 1) {
+        Request* r = group[0];
+        prove_on(r->pk, slot, r->dev_witness ? r->dev_witness : r->witness.data(), r->rs, r->proof, r->flags,
+                 nullptr);
+      } else {
+        std::vector<ProveJob> jobs;
+        for (Request* r : group)
+          jobs.push_back({r->pk, r->dev_witness ? r->dev_witness : r->witness.data(), r->rs, r->proof, r->flags});
+        prove_batch_on(jobs.data(), (int)jobs.size(), slot);
+      }
+    });
+    if (rc < 0) msg = bg_last_error();
+    return rc;
+  }
+  void release(Slot& slot) { slot.release_buffers(); }
+  uint64_t slot_bytes(const Slot& slot) { return slot.device_bytes(); }
+  void done(Request& r) { pk_release(r.pk); }  // bg_pk_free refuses the key until here
+};
 
 }  // namespace bg
+
+struct bg_engine {
+  bg::EngineOps ops;
+  std::unique_ptr<bg::sched::Scheduler<bg::EngineOps>> sched;
+  int device = 0;
+};
 
 using namespace bg;
 
@@ -215,10 +123,7 @@ extern "C" int bg_engine_create(int slots, uint64_t hbm_budget_bytes, bg_engine*
       BG_HIP(hipMemGetInfo(&fr, &tot));
       hbm_budget_bytes = (uint64_t)(fr * 0.9);  // default: 90 % of the memory free now
     }
-    e->budget = hbm_budget_bytes;
-    e->reserved.assign(slots, 0);
-    bg_engine* raw = e.get();
-    for (int i = 0; i < slots; i++) e->workers.emplace_back([raw, i] { worker_main(raw, i); });
+    e->sched = std::make_unique<sched::Scheduler<EngineOps>>(e->ops, slots, hbm_budget_bytes);
     *out = e.release();
   });
 }
@@ -244,17 +149,11 @@ extern "C" int bg_engine_submit(bg_engine* e, bg_pk* pk, const uint32_t* witness
       r->witness.assign(witness, witness + nz * ew);
     memcpy(r->rs, rs, 2 * ew * 4);
     r->need = prove_bytes_estimate(pk);
-    std::lock_guard<std::mutex> lk(e->mu);
-    BG_REQUIRE(r->need <= e->budget, BG_ERR_NOMEM,
+    BG_REQUIRE(r->need <= e->sched->budget(), BG_ERR_NOMEM,
                "bg_engine_submit: one proof of this key needs %llu bytes, budget is %llu",
-               (unsigned long long)r->need, (unsigned long long)e->budget);
-    r->ticket = e->next_ticket++;
-    r->t_submit = Clock::now();
+               (unsigned long long)r->need, (unsigned long long)e->sched->budget());
     pk_acquire(pk);  // released when the proof is done: bg_pk_free refuses the key until then
-    *ticket = r->ticket;
-    e->queue.push_back(r.get());
-    e->all[r->ticket] = std::move(r);
-    e->cv_work.notify_all();
+    *ticket = e->sched->submit(std::move(r));
   });
 }
 
@@ -262,13 +161,8 @@ extern "C" int bg_engine_wait(bg_engine* e, uint64_t ticket, uint32_t* proof_out
   std::unique_ptr<Request> r;
   int rc = guard([&] {
     BG_REQUIRE(e && proof_out, BG_ERR_ARG, "bg_engine_wait: null argument");
-    std::unique_lock<std::mutex> lk(e->mu);
-    auto it = e->all.find(ticket);
-    BG_REQUIRE(it != e->all.end(), BG_ERR_ARG, "bg_engine_wait: unknown ticket %llu", (unsigned long long)ticket);
-    Request* q = it->second.get();
-    e->cv_done.wait(lk, [&] { return q->status != 1; });
-    r = std::move(it->second);
-    e->all.erase(it);
+    r = e->sched->wait(ticket);
+    BG_REQUIRE(r != nullptr, BG_ERR_ARG, "bg_engine_wait: unknown ticket %llu", (unsigned long long)ticket);
   });
   if (rc < 0) return rc;
   if (r->status < 0) {
@@ -289,29 +183,14 @@ extern "C" int bg_engine_wait(bg_engine* e, uint64_t ticket, uint32_t* proof_out
 extern "C" int bg_engine_info(bg_engine* e, uint64_t* info4) {
   return guard([&] {
     BG_REQUIRE(e && info4, BG_ERR_ARG, "bg_engine_info: null argument");
-    std::lock_guard<std::mutex> lk(e->mu);
-    info4[0] = e->workers.size();
-    info4[1] = total_reserved(e);
-    info4[2] = e->budget;
-    info4[3] = e->completed;
+    e->sched->info(info4);
   });
 }
 
 extern "C" int bg_engine_destroy(bg_engine* e) {
   return guard([&] {
     if (!e) return;
-    {
-      std::unique_lock<std::mutex> lk(e->mu);
-      // finish what is queued or running, then stop the workers
-      e->cv_done.wait(lk, [&] {
-        for (auto& kv : e->all)
-          if (kv.second->status == 1) return false;
-        return true;
-      });
-      e->stop = true;
-    }
-    e->cv_work.notify_all();
-    for (auto& t : e->workers) t.join();
+    e->sched->shutdown();  // finishes what is queued or running, then stops the workers
     delete e;
   });
 }
