@@ -454,12 +454,16 @@ def serve_mode(a) -> dict:
         th.start()
         t_start = time.perf_counter()
         t_next = t_start
+        t_log = t_start
         for g, k in zip(gaps, kinds):
             t_next += g
             dt = t_next - time.perf_counter()
             if dt > 0:
                 time.sleep(dt)
             t_arr = time.perf_counter()
+            if t_arr - t_log >= 30.0:  # progress of a long run
+                t_log = t_arr
+                log(f"rate {rate}/s: {t_arr - t_start:.0f} s, {len(done)} proofs done")
             t = eng.submit(pks[int(k)], dz[int(k)], wz[int(k)][1])  # submit time ~ arrival time
             with cv:
                 q.append((t, int(k), t_arr))
@@ -882,6 +886,17 @@ def main() -> int:
     if not np.array_equal(bendg.pack16(p_host.reshape(-1, 16)).reshape(-1), ref):
         log("host-witness proof differs from the device-witness proof")
         return 1
+    # the same proofs with the combination and affine normalisation on the device
+    # (BG_COMBINE_DEVICE): reported beside `value`, every proof equal to the host-combined one
+    pk.prove_device(dz, rs, device_combine=True)
+    hip.sync()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        p_d, _ = pk.prove_device(dz, rs, device_combine=True)
+        if not np.array_equal(p_d, ref):
+            log("device-combined proof differs from the host-combined proof")
+            return 1
+    device_combine_ms = (time.perf_counter() - t0) * 1e3 / a.steps
     # per-kernel HIP-event times (overlapped, per proof) from separate proofs
     bendg.set_timing(True)
     bendg.timing_reset()
@@ -923,6 +938,7 @@ def main() -> int:
         "latency_ms": {"p50": round(float(np.percentile(walls, 50)), 3),
                        "p99": round(float(np.percentile(walls, 99)), 3)},
         "pcie_inclusive_ms": round(pcie_ms, 3),
+        "device_combine_ms": round(device_combine_ms, 3),
         "pcie_inclusive_latency_ms": {"p50": round(float(np.percentile(pcie_walls, 50)), 3),
                                       "p99": round(float(np.percentile(pcie_walls, 99)), 3)},
         "proof_verified": verified,

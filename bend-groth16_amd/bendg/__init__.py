@@ -28,6 +28,7 @@ FMT_WORDS16 = 0
 FMT_PACKED = 1
 MEM_HOST = 0
 MEM_DEVICE = 2
+COMBINE_DEVICE = 4  # bg_prove: combination and affine proof points on the device
 
 NTT_FFT, NTT_IFFT, NTT_COSET_FFT, NTT_COSET_IFFT = 0, 1, 2, 3
 

@@ -8,7 +8,7 @@ from pathlib import Path
 
 import numpy as np
 
-from . import FMT_PACKED, FMT_WORDS16, _check, _p, lib
+from . import COMBINE_DEVICE, FMT_PACKED, FMT_WORDS16, _check, _p, lib
 
 PHASES = ("upload", "evals", "qap", "msm_a", "msm_b1", "msm_b2", "msm_h", "msm_l", "total")
 
@@ -40,20 +40,23 @@ class ProvingKey:
         _check(lib().bg_pk_device_bytes(self._h, _p(b)))
         return int(b[0])
 
-    def prove_device(self, witness_ptr: int, rs: np.ndarray):
+    def prove_device(self, witness_ptr: int, rs: np.ndarray, device_combine: bool = False):
         """Proof with the packed witness already resident in HBM (device pointer);
-        rs is host (packed).  Returns (packed proof, phase times)."""
+        rs is host (packed).  device_combine: the combination and affine points on the
+        device (BG_COMBINE_DEVICE).  Returns (packed proof, phase times)."""
         r = np.ascontiguousarray(rs, dtype=np.uint32).reshape(-1)
         if r.size != 16:
             raise ValueError("rs must hold two packed Fr elements")
         out = np.zeros(64, dtype=np.uint32)
         t = np.zeros(9, dtype=np.float64)
-        _check(lib().bg_prove(self._h, ctypes.c_void_p(witness_ptr), _p(r), _p(out), FMT_PACKED | 2, _p(t)))
+        flags = FMT_PACKED | 2 | (COMBINE_DEVICE if device_combine else 0)
+        _check(lib().bg_prove(self._h, ctypes.c_void_p(witness_ptr), _p(r), _p(out), flags, _p(t)))
         return out, dict(zip(PHASES, (float(x) for x in t)))
 
-    def prove(self, witness: np.ndarray, rs: np.ndarray, fmt: str = "words16"):
+    def prove(self, witness: np.ndarray, rs: np.ndarray, fmt: str = "words16", device_combine: bool = False):
         """witness / rs in FORMAT.md words (``words16``) or packed u32; returns
-        (proof uint32 array [A | B | C] in the same encoding, phase times dict)."""
+        (proof uint32 array [A | B | C] in the same encoding, phase times dict).
+        device_combine: the combination and affine points on the device."""
         ew = 16 if fmt == "words16" else 8
         w = np.ascontiguousarray(witness, dtype=np.uint32).reshape(-1)
         r = np.ascontiguousarray(rs, dtype=np.uint32).reshape(-1)
@@ -64,7 +67,7 @@ class ProvingKey:
             raise ValueError(f"witness must hold {n_z} Fr elements")
         out = np.zeros(8 * ew, dtype=np.uint32)
         t = np.zeros(9, dtype=np.float64)
-        flag = FMT_WORDS16 if fmt == "words16" else FMT_PACKED
+        flag = (FMT_WORDS16 if fmt == "words16" else FMT_PACKED) | (COMBINE_DEVICE if device_combine else 0)
         _check(lib().bg_prove(self._h, _p(w), _p(r), _p(out), flag, _p(t)))
         return out, dict(zip(PHASES, (float(x) for x in t)))
 

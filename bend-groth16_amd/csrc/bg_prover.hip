@@ -131,6 +131,9 @@ struct bg_pk {
   // plain arrays (windowed MSMs).
   bool fixed = false;
   int c_g1 = 0, c_h = 0, c_b2 = 0;
+  // 4-bit-window tables of delta (G1) and delta2 (G2): entry 16 w + j = j 2^(4w) delta,
+  // XYZZ, for the device combination's r delta, s delta, rs delta and s delta2
+  bg::DevArray dtab1, dtab2;
   uint64_t dev_bytes = 0;  // resident key bytes
   // slot used by bg_prove (created on first use; one bg_prove call per key at a time)
   std::unique_ptr<bg::ProveSlot> own;
@@ -140,6 +143,27 @@ struct bg_pk {
 };
 
 namespace bg {
+
+// 64 x 16 XYZZ points j 2^(4w) P (w < 64, j < 16) of an affine point, packed words
+template <class E>
+static std::vector<uint32_t> window_table(const uint32_t* aff) {
+  const int k = bgh::words<E>();
+  std::vector<uint32_t> out((size_t)64 * 16 * 4 * k);
+  bgh::Pt<E> base = bgh::load_affine<E>(aff);
+  for (int w = 0; w < 64; w++) {
+    bgh::Pt<E> acc = bgh::Pt<E>::inf();
+    for (int j = 0; j < 16; j++) {
+      uint32_t* o = &out[(size_t)(16 * w + j) * 4 * k];
+      acc.X.to_u32(o);
+      acc.Y.to_u32(o + k);
+      acc.ZZ.to_u32(o + 2 * k);
+      acc.ZZZ.to_u32(o + 3 * k);
+      acc = acc.add(base);
+    }
+    base = base.dbl_n(4);
+  }
+  return out;
+}
 
 static void upload_array(DevArray& d, const void* src, size_t bytes, uint64_t& total) {
   d.alloc(bytes);
@@ -311,6 +335,12 @@ extern "C" int bg_pk_load(const char* dir, bg_pk** out) {
         BG_HIP(hipStreamSynchronize(c.stream));
       }
     }
+    {
+      auto t1 = window_table<bgh::Fq>(pk->g1 + 32);
+      auto t2 = window_table<bgh::Fq2>(pk->g2 + 32);
+      upload_array(pk->dtab1, t1.data(), t1.size() * 4, total);
+      upload_array(pk->dtab2, t2.data(), t2.size() * 4, total);
+    }
     pk->dev_bytes = total;
     (void)ntt_domain((int)lg);  // twiddle / scale tables resident with the key
     *out = pk.release();
@@ -382,14 +412,15 @@ void ProveSlot::ensure(uint64_t nz, uint64_t d) {
 }
 
 void ProveSlot::release_buffers() {
-  for (DevArray* a : {&z, &zc, &ea, &eb, &ec, &h, &tmp}) a->release();
+  for (DevArray* a : {&z, &zc, &ea, &eb, &ec, &h, &tmp, &comb}) a->release();
   zpin.release();
+  proofpin.release();
   cap_nz = cap_d = 0;
   for (auto& w : work) w.release();
 }
 
 uint64_t ProveSlot::device_bytes() const {
-  uint64_t t = 64 * cap_nz + 160 * cap_d;
+  uint64_t t = 64 * cap_nz + 160 * cap_d + comb.bytes;
   for (auto& w : work) t += w.bytes();
   return t;
 }
@@ -525,6 +556,37 @@ void prove_on(bg_pk* pk, ProveSlot& sl, const uint32_t* witness, const uint32_t*
     if (sched == 5) s1 = s2 = s3 = s0;  // one stream: every kernel alone (profiling)
     hipEvent_t* ev = sl.ev;
     sl.ensure(nz, d);
+    // r, s and r s canonical: the scalars of the combination's delta terms and products
+    uint64_t rc[4], sc[4], rsc[4];
+    canon_words(rs_packed, rc);
+    canon_words(rs_packed + 8, sc);
+    {
+      bgh::Fr rs_m = bgh::Fr::from_u32(rs_packed) * bgh::Fr::from_u32(rs_packed + 8);
+      bgh::Fr x = rs_m.from_mont();
+      memcpy(rsc, x.v, 32);
+    }
+    // the combination on the device (BG_COMBINE_DEVICE): its delta terms need only r and s,
+    // so they run from the start beside everything else
+    const bool dev_comb = (flags & BG_COMBINE_DEVICE) != 0;
+    BG_REQUIRE(!dev_comb || pk->fixed, BG_ERR_ARG, "bg_prove: the device combination needs fixed-base tables");
+    CombineArgs ca{};
+    uint32_t* cb = nullptr;
+    if (dev_comb) {
+      if (!sl.comb.ptr) sl.comb.alloc(4 * comb_words());
+      cb = sl.comb.as<uint32_t>();
+      memcpy(ca.alpha1, pk->g1, 64);
+      memcpy(ca.beta1, pk->g1 + 16, 64);
+      memcpy(ca.delta1, pk->g1 + 32, 64);
+      memcpy(ca.beta2, pk->g2, 128);
+      memcpy(ca.delta2, pk->g2 + 32, 128);
+      memcpy(ca.sc, rc, 32);
+      memcpy(ca.sc + 8, sc, 32);
+      memcpy(ca.sc + 16, rsc, 32);
+      ca.dtab1 = pk->dtab1.as<uint32_t>();
+      ca.dtab2 = pk->dtab2.as<uint32_t>();
+      launch_delta_terms(ca, cb, s2);
+      BG_HIP(hipEventRecord(ev[12], s2));
+    }
     BG_HIP(hipEventRecord(ev[0], s0));
     // 1. witness upload (a host witness is packed by the worker pool into pinned staging
     //    and sent in one copy; a device witness is copied in place)
@@ -630,15 +692,59 @@ void prove_on(bg_pk* pk, ProveSlot& sl, const uint32_t* witness, const uint32_t*
     }
     BG_HIP(hipEventRecord(ev[11], s0));
 
-    // host work that depends only on r, s overlaps with the device
-    uint64_t rc[4], sc[4], rsc[4];
-    canon_words(rs_packed, rc);
-    canon_words(rs_packed + 8, sc);
-    {
-      bgh::Fr rs_m = bgh::Fr::from_u32(rs_packed) * bgh::Fr::from_u32(rs_packed + 8);
-      bgh::Fr x = rs_m.from_mont();
-      memcpy(rsc, x.v, 32);
+    if (dev_comb) {
+      // A, B1, L finished and combined once the G1 batch is reduced (its stream), B2 on its
+      // stream, C after MSM H on s0; only the affine proof comes back
+      auto shape = [](const MsmWork& w, uint32_t& hn, int& log2m) {
+        hn = w.plan.H;
+        log2m = 0;
+        while ((1u << log2m) < w.plan.m * w.plan.K) log2m++;
+      };
+      shape(sl.work[0], ca.hn_g1, ca.log2m_g1);
+      shape(sl.work[1], ca.hn_h, ca.log2m_h);
+      shape(sl.work[2], ca.hn_b2, ca.log2m_b2);
+      BG_REQUIRE(sl.work[0].plan.W == 1 && sl.work[1].plan.W == 1 && sl.work[2].plan.W == 1 &&
+                     sl.work[0].plan.M == 3, BG_ERR_INTERNAL, "bg_prove: unexpected MSM shape for the device combination");
+      const hipStream_t sg1 = sched == 1 ? s0 : s1;  // the G1 batch's stream
+      BG_HIP(hipStreamWaitEvent(sg1, ev[12], 0));
+      launch_combine_g1(ca, static_cast<const uint32_t*>(sl.work[0].outPQ.ptr), cb, sg1);
+      BG_HIP(hipEventRecord(ev[13], sg1));  // C0 ready
+      launch_affine_a(cb, sg1);             // A's normalisation, off C's path
+      BG_HIP(hipEventRecord(ev[7], sg1));
+      BG_HIP(hipStreamWaitEvent(s3, ev[12], 0));
+      launch_combine_b2(ca, static_cast<const uint32_t*>(sl.work[2].outPQ.ptr), cb, s3);
+      BG_HIP(hipEventRecord(ev[14], s3));
+      BG_HIP(hipStreamWaitEvent(s0, ev[13], 0));
+      BG_HIP(hipStreamWaitEvent(s0, ev[14], 0));
+      launch_combine_final(ca, static_cast<const uint32_t*>(sl.work[1].outPQ.ptr), cb, s0);
+      BG_HIP(hipStreamWaitEvent(s0, ev[7], 0));
+      uint32_t* pin = static_cast<uint32_t*>(sl.proofpin.get(256));
+      BG_HIP(hipMemcpyAsync(pin, cb + comb_proof_offset(), 256, hipMemcpyDeviceToHost, s0));
+      BG_HIP(hipEventRecord(ev[15], s0));
+      BG_HIP(hipEventSynchronize(ev[15]));
+      uint32_t proof[64];
+      memcpy(proof, pin, sizeof proof);
+      if (words16) {
+        for (int i = 0; i < 64; i++) {
+          proof_out[2 * i] = proof[i] & 0xFFFFu;
+          proof_out[2 * i + 1] = proof[i] >> 16;
+        }
+      } else {
+        memcpy(proof_out, proof, sizeof proof);
+      }
+      if (t_ms) {
+        const int pairs[8][2] = {{0, 1}, {1, 3}, {3, 4}, {5, 6}, {5, 6}, {9, 10}, {4, 11}, {5, 6}};
+        for (int i = 0; i < 8; i++) {
+          float e = 0;
+          BG_HIP(hipEventElapsedTime(&e, ev[pairs[i][0]], ev[pairs[i][1]]));
+          t_ms[i] = e;
+        }
+        t_ms[8] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      }
+      return;
     }
+
+    // host work that depends only on r, s overlaps with the device
     bgh::P1 alpha = bgh::load_affine<bgh::Fq>(pk->g1), beta1 = bgh::load_affine<bgh::Fq>(pk->g1 + 16),
        delta1 = bgh::load_affine<bgh::Fq>(pk->g1 + 32);
     bgh::P2 beta2 = bgh::load_affine<bgh::Fq2>(pk->g2), delta2 = bgh::load_affine<bgh::Fq2>(pk->g2 + 32);

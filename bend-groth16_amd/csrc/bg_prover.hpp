@@ -15,7 +15,9 @@ struct ProveSlot {
   hipStream_t s[4] = {};
   hipEvent_t ev[16] = {};
   DevArray z, zc, ea, eb, ec, h, tmp;
-  HostBuf zpin;  // pinned staging of a host witness (packed words)
+  DevArray comb;  // the device combination's scratch (BG_COMBINE_DEVICE)
+  HostBuf zpin;   // pinned staging of a host witness (packed words)
+  HostBuf proofpin;  // pinned landing of a device-combined proof
   uint64_t cap_nz = 0, cap_d = 0;
   MsmWork work[3];  // G1 batch (A, B1, L), H, B2
   ProveSlot();
@@ -37,6 +39,25 @@ void check_host_inputs(const uint32_t* witness, const uint32_t* rs, int flags, u
 // Proofs of a key that are queued or running: bg_pk_free refuses a key while any are.
 void pk_acquire(bg_pk* pk);
 void pk_release(bg_pk* pk);
+
+// Final combination on the device (BG_COMBINE_DEVICE, bg_combine.hip): the key's points,
+// r, s, rs (canonical) and the shapes of the three MSM batches' level-2 outputs.
+struct CombineArgs {
+  uint32_t alpha1[16], beta1[16], delta1[16];  // packed affine Montgomery
+  uint32_t beta2[32], delta2[32];
+  uint32_t sc[24];                              // r, s, rs canonical
+  const uint32_t* dtab1;                        // j 2^(4w) delta (G1 XYZZ, w < 64, j < 16)
+  const uint32_t* dtab2;                        // the same for delta2 (G2)
+  uint32_t hn_g1, hn_h, hn_b2;                  // level-2 outputs per MSM (powers of two)
+  int log2m_g1, log2m_h, log2m_b2;              // log2(m K) of each batch's plan
+};
+uint64_t comb_words();         // device words of one proof's combination scratch
+uint32_t comb_proof_offset();  // where the proof (64 packed words) lands in it
+void launch_delta_terms(const CombineArgs& a, uint32_t* cb, hipStream_t s);
+void launch_combine_g1(const CombineArgs& a, const uint32_t* outPQ, uint32_t* cb, hipStream_t s);
+void launch_affine_a(uint32_t* cb, hipStream_t s);
+void launch_combine_b2(const CombineArgs& a, const uint32_t* outPQ, uint32_t* cb, hipStream_t s);
+void launch_combine_final(const CombineArgs& a, const uint32_t* outPQ, uint32_t* cb, hipStream_t s);
 
 // Runs one proof on slot sl (blocking until the proof is on the host).
 void prove_on(bg_pk* pk, ProveSlot& sl, const uint32_t* witness, const uint32_t* rs, uint32_t* proof_out, int flags,

@@ -40,7 +40,11 @@ enum {
   BG_FMT_WORDS16 = 0,
   BG_FMT_PACKED = 1,
   BG_MEM_HOST = 0,
-  BG_MEM_DEVICE = 2
+  BG_MEM_DEVICE = 2,
+  /* bg_prove only: finish the MSMs and form the proof's combination and affine points on
+   * the device (bend2/src/groth16.bend:281-308 there, no host round trip); needs a key
+   * with fixed-base tables.  The default combines on the host while MSM H runs. */
+  BG_COMBINE_DEVICE = 4
 };
 
 /* NTT directions: bend2/src/ntt.bend:431-442 (fft, ifft, coset_fft, coset_ifft) */

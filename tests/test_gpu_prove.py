@@ -209,3 +209,40 @@ def test_cli_schedules(bg, keys, sched):
     assert ref.returncode == 0 and out.returncode == 0, out.stderr
     assert out.stdout.split("\n")[:3] == ref.stdout.split("\n")[:3]
     assert out.stdout.strip().split("\n")[-1] == "V ok"
+
+
+@pytest.mark.parametrize("lg", [4, 8, 10, 12, 16])
+def test_device_combination_equals_host(bg, tmp_path, lg):
+    """BG_COMBINE_DEVICE (MSM finishing, groth16.bend's combination and the affine
+    normalisation on the device) gives the proof the host combination gives, bit for bit,
+    in both encodings and on repeated calls; the K=12 proof also equals the closed form."""
+    d = tmp_path / f"k{lg}"
+    bendg.setup_square_chain(d, lg, seed=500 + lg)
+    z, rs = _files(d)
+    pk = bendg.ProvingKey(d)
+    host16, _ = pk.prove(z, rs, fmt="words16")
+    print(f"device combination K={lg}", flush=True)
+    dev16, _ = pk.prove(z, rs, fmt="words16", device_combine=True)
+    assert np.array_equal(dev16, host16)
+    zp = bendg.pack16(z.reshape(-1, 16)).reshape(-1)
+    rp = bendg.pack16(rs.reshape(-1, 16)).reshape(-1)
+    for _ in range(2):
+        devp, _ = pk.prove(zp, rp, fmt="packed", device_combine=True)
+        assert np.array_equal(devp, bendg.pack16(host16.reshape(-1, 16)).reshape(-1))
+    if lg == 12:
+        eA, eB, eC = points_of(closed_form(load(d)))
+        A, B, C = _split(devp)
+        assert np.array_equal(A, eA) and np.array_equal(B, eB) and np.array_equal(C, eC)
+    pk.close()
+
+
+def test_device_combination_needs_fixed_base(bg, keys, monkeypatch):
+    d = keys[4]
+    monkeypatch.setenv("BG16_FIXED_BASE", "0")
+    pk = bendg.ProvingKey(d)
+    monkeypatch.delenv("BG16_FIXED_BASE")
+    z, rs = _files(d)
+    with pytest.raises(bendg.BgError) as ei:
+        pk.prove(z, rs, device_combine=True)
+    assert ei.value.code == -1
+    pk.close()
