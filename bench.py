@@ -32,6 +32,8 @@ Other modes (one JSON line each):
                      tile and radix of the NTT on the device and writes the parameter table
                      the planners read (tune_gfx950.txt).
   --mode roofline    the standalone MSM pass and the transforms alone (profiled by rocprofv3).
+  --mode sizes       single proofs at K = 10..18, idle gaps between sizes (for a kernel trace of
+                     the single path's device-busy fraction by size).
 """
 from __future__ import annotations
 
@@ -640,6 +642,38 @@ def roofline_mode(a) -> dict:
             "transform_ms": transforms}
 
 
+def sizes_mode(a) -> dict:
+    """One proof at a time at K = 10, 12, 14, 16, 18 (`steps` timed proofs each, witnesses
+    resident), 100 ms of idle device between sizes so that a kernel trace of this run splits
+    by size: with rocprofv3 --kernel-trace, the union of the kernels' intervals over a size's
+    proofs against their wall time is the single path's device-busy fraction."""
+    bendg.init(0)
+    hip = Hip()
+    out = {}
+    for k in (10, 12, 14, 16, 18):
+        kdir = ensure_key(k, 42 + k, Path(a.keys))
+        pk = bendg.ProvingKey(kdir)
+        z, rs = witness_files(kdir)
+        dz = hip.upload(z)
+        for _ in range(3):
+            pk.prove_device(dz, rs)
+        hip.sync()
+        time.sleep(0.1)
+        walls = []
+        for _ in range(a.steps):
+            t0 = time.perf_counter()
+            pk.prove_device(dz, rs)
+            walls.append((time.perf_counter() - t0) * 1e3)
+        hip.sync()
+        time.sleep(0.1)
+        out[str(k)] = {"median_ms": round(float(np.median(walls)), 3), "proofs": a.steps}
+        log(f"K={k}: {out[str(k)]}")
+        hip.free(dz)
+        pk.close()
+    return {"metric": "single-proof wall time by size", "value": out["18"]["median_ms"], "unit": "ms/proof",
+            "n_gpus": 1, "sizes": out}
+
+
 def tune_mode(a) -> dict:
     """Device-measured parameter table (replaces Msm.pick_c / pick_fd, bend2/src/msm.bend:37-44,
     and Ntt.fd, bend2/src/ntt.bend:382-385).  Each candidate runs with a one-entry table (plus
@@ -808,7 +842,7 @@ def main() -> int:
     ap.add_argument("--keys", default=os.environ.get("BG16_KEYS", "/tmp/bg16_keys"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mix", action="store_true", help="single mode: skip the 64-proof batched mix")
-    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve", "primitives", "roofline", "tune"])
+    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve", "primitives", "roofline", "tune", "sizes"])
     ap.add_argument("--slots", type=lambda v: [int(x) for x in v.split(",")], default=[4, 8],
                     help="engine slots (batch: comma list to sweep)")
     ap.add_argument("--rates", type=lambda v: [float(x) for x in v.split(",")], default=[100.0, 200.0, 300.0],
@@ -828,7 +862,7 @@ def main() -> int:
         os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, max(1, hwq)))
     if a.mode != "single":
         line = {"batch": batch_mode, "serve": serve_mode, "primitives": primitives_mode,
-                "roofline": roofline_mode, "tune": tune_mode}[a.mode](a)
+                "roofline": roofline_mode, "tune": tune_mode, "sizes": sizes_mode}[a.mode](a)
         print(json.dumps(line), flush=True)
         return 0
 
