@@ -4,6 +4,7 @@
 #pragma once
 #include "bg_msm.hpp"
 #include "bg_curve.hpp"
+#include "bg_curve_fd.hpp"
 
 namespace bg {
 
@@ -32,6 +33,97 @@ __device__ __forceinline__ const uint32_t* entry_points(const uint64_t* __restri
   return reinterpret_cast<const uint32_t*>(ptrs[key / kspan]);
 }
 
+// The running sum of a G1 run lives in the digit domain (bg_curve_fd.hpp); points are
+// read and buckets written in the word domain.  G2 keeps the word-domain formula: its
+// digit-domain state (four Fq2 coordinates of 18 digits plus the operands of the
+// one-reduction Fq2 products) does not fit two waves' registers per SIMD without spills.
+template <class F>
+struct WordAcc {
+  using C = Curve<F>;
+  using Pt = typename C::P;
+  using Raw = typename C::A;
+  static constexpr int aff_words = C::aff_words, xyzz_words = C::xyzz_words;
+  static constexpr int ew = C::elem_words;
+  __device__ __forceinline__ static typename F::E elem(const uint32_t* w) {
+    typename F::E e;
+    uint32_t* d = reinterpret_cast<uint32_t*>(&e);
+#pragma unroll
+    for (int i = 0; i < ew; i++) d[i] = w[i];
+    return e;
+  }
+  __device__ __forceinline__ static void put(uint32_t* w, const typename F::E& e) {
+    const uint32_t* d = reinterpret_cast<const uint32_t*>(&e);
+#pragma unroll
+    for (int i = 0; i < ew; i++) w[i] = d[i];
+  }
+  __device__ __forceinline__ static Raw load(const uint32_t* w) { return {elem(w), elem(w + ew)}; }
+  __device__ __forceinline__ static Pt start(const Raw& a, bool neg) {
+    return {a.x, neg ? F::neg(a.y) : a.y, F::one(), F::one()};
+  }
+  __device__ __forceinline__ static Pt madd(const Pt& p, Raw a, bool neg, uint32_t& bad) {
+    if (neg) a.y = F::neg(a.y);
+    return C::madd_generic(p, a, bad);
+  }
+  __device__ __forceinline__ static void store(uint32_t* w, const Pt& p) {
+    put(w, p.X);
+    put(w + ew, p.Y);
+    put(w + 2 * ew, p.ZZ);
+    put(w + 3 * ew, p.ZZZ);
+  }
+};
+
+template <class F>
+struct AccOf;
+template <>
+struct AccOf<Field<FqP>> {
+  using type = fd::G1Acc<fd::FqDig>;
+};
+template <>
+struct AccOf<Fq2> {
+  using type = WordAcc<Fq2>;
+};
+
+constexpr bool same_digits(const uint32_t* a, const uint32_t* b) {
+  for (int i = 0; i < 9; i++)
+    if (a[i] != b[i]) return false;
+  return true;
+}
+static_assert(same_digits(FqP::p29, fd::FqDig::p29) && FqP::n0_29 == fd::FqDig::n0_29 &&
+                  FqP::p2[7] == fd::FqDig::top2p,
+              "digit-domain constants of Fq");
+
+// W words (a multiple of 4, 16-byte aligned) between global memory and registers
+template <int W>
+__device__ __forceinline__ void load_vec(const uint32_t* __restrict__ p, uint32_t* w) {
+  const uint4* q = reinterpret_cast<const uint4*>(p);
+#pragma unroll
+  for (int i = 0; i < W / 4; i++) {
+    const uint4 v = q[i];
+    w[4 * i] = v.x;
+    w[4 * i + 1] = v.y;
+    w[4 * i + 2] = v.z;
+    w[4 * i + 3] = v.w;
+  }
+}
+template <int W>
+__device__ __forceinline__ void store_vec(uint32_t* __restrict__ p, const uint32_t* w) {
+  uint4* q = reinterpret_cast<uint4*>(p);
+#pragma unroll
+  for (int i = 0; i < W / 4; i++) q[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+}
+template <class A>
+__device__ __forceinline__ typename A::Raw load_point(const uint32_t* __restrict__ p) {
+  uint32_t w[A::aff_words];
+  load_vec<A::aff_words>(p, w);
+  return A::load(w);
+}
+template <class A>
+__device__ __forceinline__ void store_acc(uint32_t* __restrict__ p, const typename A::Pt& acc) {
+  uint32_t w[A::xyzz_words];
+  A::store(w, acc);
+  store_vec<A::xyzz_words>(p, w);
+}
+
 template <class F>
 __global__ __launch_bounds__(256, 2) void k_accumulate(const uint64_t* __restrict__ ptrs, uint32_t kspan,
                                                     const uint2* __restrict__ sorted,
@@ -41,6 +133,8 @@ __global__ __launch_bounds__(256, 2) void k_accumulate(const uint64_t* __restric
                                                     uint32_t* __restrict__ ppts, uint32_t nb,
                                                     uint32_t* __restrict__ fix) {
   using C = Curve<F>;
+  using A = typename AccOf<F>::type;
+  static_assert(A::aff_words == C::aff_words && A::xyzz_words == C::xyzz_words, "point layout");
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nthreads) return;
   pkeys[2 * t] = NOKEY_C;
@@ -51,47 +145,46 @@ __global__ __launch_bounds__(256, 2) void k_accumulate(const uint64_t* __restric
   uint64_t end = min(start + L, (uint64_t)T);
   uint32_t prevkey = start > 0 ? sorted[start - 1].x : NOKEY_C;
   uint32_t nextkey = end < T ? sorted[end].x : NOKEY_C;
-  uint2 e0 = sorted[start];
-  uint32_t cur = e0.x;
-  const uint32_t* points = entry_points(ptrs, cur, kspan);
-  typename C::A p0 = C::load_aff(points + (uint64_t)C::aff_words * (e0.y & 0x7FFFFFFFu));
-  if (e0.y >> 31) p0.y = F::neg(p0.y);
-  typename C::P acc = {p0.x, p0.y, F::one(), F::one()};
+  // One point load, one run start, one addition and one store site in the loop: the
+  // digit-domain conversions at a run's start and end are whole products, so each is
+  // emitted once (a smaller kernel for the instruction cache).  Step e == end closes the
+  // last run.
+  uint32_t cur = NOKEY_C;
+  const uint32_t* points = nullptr;
   bool first = true;
   uint32_t bad = 0;
-  for (uint64_t e = start + 1; e < end; e++) {
-    uint2 ent = sorted[e];
-    if (ent.x / kspan != cur / kspan) points = entry_points(ptrs, ent.x, kspan);
-    typename C::A pt = C::load_aff(points + (uint64_t)C::aff_words * (ent.y & 0x7FFFFFFFu));
-    if (ent.y >> 31) pt.y = F::neg(pt.y);
-    if (ent.x != cur) {
-      bool cp = first && prevkey == cur;
+  typename A::Pt acc;
+  for (uint64_t e = start;; e++) {
+    const bool last = e == end;
+    const uint2 ent = last ? make_uint2(NOKEY_C, 0u) : sorted[e];
+    if (e > start && ent.x != cur) {
+      // the run of key cur ends: a complete bucket, or a partial for the merge
+      const bool cp = first && prevkey == cur;
+      const bool cn = last && nextkey == cur;
       if (bad) flag_bucket<F>(cur, nb, fix);
-      if (!cp) {
-        C::store(buckets + (uint64_t)C::xyzz_words * cur, acc);
+      uint32_t* dst;
+      if (!cp && !cn) {
+        dst = buckets + (uint64_t)A::xyzz_words * cur;
       } else {
-        pkeys[2 * t] = cur;
-        pflags[2 * t] = 1;
-        C::store(ppts + (uint64_t)C::xyzz_words * (2 * t), acc);
+        const uint32_t slot = cn ? 2 * t + 1 : 2 * t;
+        pkeys[slot] = cur;
+        pflags[slot] = cp ? 1u : 0u;
+        dst = ppts + (uint64_t)A::xyzz_words * slot;
       }
+      store_acc<A>(dst, acc);
       first = false;
+    }
+    if (last) break;
+    if (e == start || ent.x / kspan != cur / kspan) points = entry_points(ptrs, ent.x, kspan);
+    const typename A::Raw pt = load_point<A>(points + (uint64_t)A::aff_words * (ent.y & 0x7FFFFFFFu));
+    const bool neg = (ent.y >> 31) != 0;
+    if (ent.x != cur) {
       cur = ent.x;
       bad = 0;
-      acc = {pt.x, pt.y, F::one(), F::one()};
+      acc = A::start(pt, neg);
     } else {
-      acc = C::madd_generic(acc, pt, bad);
+      acc = A::madd(acc, pt, neg, bad);
     }
-  }
-  bool cp = first && prevkey == cur;
-  bool cn = nextkey == cur;
-  if (bad) flag_bucket<F>(cur, nb, fix);
-  if (!cp && !cn) {
-    C::store(buckets + (uint64_t)C::xyzz_words * cur, acc);
-  } else {
-    uint32_t slot = cn ? 2 * t + 1 : 2 * t;
-    pkeys[slot] = cur;
-    pflags[slot] = cp ? 1u : 0u;
-    C::store(ppts + (uint64_t)C::xyzz_words * slot, acc);
   }
 }
 
