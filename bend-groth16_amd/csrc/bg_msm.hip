@@ -462,6 +462,209 @@ __global__ void k_scatter(const uint32_t* __restrict__ keys, const uint32_t* __r
   sorted[pos + (rk & 0xFFFFFFu)] = make_uint2(key, val);
 }
 
+// ------------------------------------------------------------------ two-pass bucket sort
+// The entries (key, val) of an MSM batch ordered by key in two counting passes over the
+// key bits, each of whose scatters writes a block's entries of one bin to one contiguous
+// range (the single-pass sort above writes every entry to a random 8-byte slot and reads a
+// random histogram word for it):
+//   pass 1, by the high key bits (at most 2^14 bins): a block takes 1024 scalars of one
+//     MSM, forms all their signed digits, counts them in an LDS histogram, and after a
+//     scan of the (bin, block) counts writes each entry to its bin's range;
+//   pass 2, by the low 8 key bits inside each high-bits segment: tiles of 4096 entries of
+//     one segment, an LDS histogram each, a scan of the (segment, bin, tile) counts, and
+//     the scatter into the final order.
+// Ranks within a bin come from LDS atomics, so the order of a bucket's entries is not
+// fixed; a bucket's sum does not depend on it (the accumulation's special cases are exact).
+static constexpr int SORT_T = 256;
+static constexpr uint32_t P1_SCALARS = 1024;  // scalars per pass-1 block
+static constexpr int P1_MAX_BITS = 14;        // pass-1 bins: a 64 KB LDS histogram
+static constexpr int P2_MAX_BITS = 8;
+static constexpr uint32_t P2_TILE = 4096;     // entries per pass-2 block
+
+// Signed c-bit digits of a reduced scalar, low window first: digit_w = bits_w + carry_w,
+// minus 2^c (carry 1 into window w+1) when it exceeds 2^(c-1); fn(w, bucket, negative) for
+// each nonzero digit (the same digits as k_count's closed-form carry).
+template <class Fn>
+__device__ __forceinline__ void for_each_digit(const uint32_t* s, int c, int Wd, Fn fn) {
+  const uint32_t B = 1u << (c - 1), full = 1u << c;
+  uint32_t carry = 0;
+  for (int w = 0; w < Wd; w++) {
+    const int lo = w * c;
+    const uint32_t v = (lo < 256 ? get_bits(s, lo, c) : 0u) + carry;
+    carry = v > B ? 1u : 0u;
+    if (v != 0 && v != full) {
+      const bool neg = v > B;
+      fn(w, (neg ? full - v : v) - 1u, neg);
+    }
+  }
+}
+
+struct SortArgs {
+  const uint32_t* sc;  // reduced scalars of this MSM
+  uint32_t n, key_base, blk_base, nb1_total, nbins1;
+  int c, Wd, fixed, hb;
+};
+
+__device__ __forceinline__ void load_scalar(const uint32_t* __restrict__ sc, uint64_t j, uint32_t* s) {
+  const uint4* q = reinterpret_cast<const uint4*>(sc + 8 * j);
+  const uint4 a = q[0], b = q[1];
+  s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w;
+  s[4] = b.x; s[5] = b.y; s[6] = b.z; s[7] = b.w;
+}
+
+__global__ __launch_bounds__(SORT_T) void k_sort_p1_count(SortArgs a, uint32_t* __restrict__ hist1) {
+  extern __shared__ uint32_t h1[];
+  for (uint32_t b = threadIdx.x; b < a.nbins1; b += SORT_T) h1[b] = 0;
+  __syncthreads();
+  const uint32_t B = 1u << (a.c - 1);
+  const uint64_t j0 = (uint64_t)blockIdx.x * P1_SCALARS, j1 = min(j0 + P1_SCALARS, (uint64_t)a.n);
+  for (uint64_t j = j0 + threadIdx.x; j < j1; j += SORT_T) {
+    uint32_t s[8];
+    load_scalar(a.sc, j, s);
+    for_each_digit(s, a.c, a.Wd, [&](int w, uint32_t bk, bool) {
+      const uint32_t key = a.key_base + (a.fixed ? 0u : (uint32_t)w * B) + bk;
+      atomicAdd(&h1[key >> a.hb], 1u);
+    });
+  }
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < a.nbins1; b += SORT_T)
+    hist1[(uint64_t)b * a.nb1_total + a.blk_base + blockIdx.x] = h1[b];
+}
+
+// base1: the exclusive scan of hist1 (absolute first positions of each (bin, block))
+__global__ __launch_bounds__(SORT_T) void k_sort_p1_scatter(SortArgs a, const uint32_t* __restrict__ base1,
+                                                            uint2* __restrict__ tmp) {
+  extern __shared__ uint32_t cur1[];
+  for (uint32_t b = threadIdx.x; b < a.nbins1; b += SORT_T)
+    cur1[b] = base1[(uint64_t)b * a.nb1_total + a.blk_base + blockIdx.x];
+  __syncthreads();
+  const uint32_t B = 1u << (a.c - 1);
+  const uint64_t j0 = (uint64_t)blockIdx.x * P1_SCALARS, j1 = min(j0 + P1_SCALARS, (uint64_t)a.n);
+  for (uint64_t j = j0 + threadIdx.x; j < j1; j += SORT_T) {
+    uint32_t s[8];
+    load_scalar(a.sc, j, s);
+    for_each_digit(s, a.c, a.Wd, [&](int w, uint32_t bk, bool neg) {
+      const uint32_t key = a.key_base + (a.fixed ? 0u : (uint32_t)w * B) + bk;
+      const uint32_t val = (a.fixed ? (uint32_t)w * a.n + (uint32_t)j : (uint32_t)j) | (neg ? 0x80000000u : 0u);
+      tmp[atomicAdd(&cur1[key >> a.hb], 1u)] = make_uint2(key, val);
+    });
+  }
+}
+
+// One block: segment starts (column 0 of the scanned pass-1 counts; seg[nbins1] = total),
+// tiles per segment and their exclusive scan (wbase[nbins1] = tiles in all).
+__global__ __launch_bounds__(1024) void k_sort_tiles(const uint32_t* __restrict__ base1, uint32_t nb1_total,
+                                                     uint32_t nbins1, const uint32_t* __restrict__ total,
+                                                     uint32_t* __restrict__ seg, uint32_t* __restrict__ ntiles,
+                                                     uint32_t* __restrict__ wbase) {
+  __shared__ uint32_t sh[1024];
+  const uint32_t t = threadIdx.x;
+  const uint32_t per = (nbins1 + 1023) / 1024, lo = t * per, hi = min(lo + per, nbins1);
+  const uint32_t tot = *total;
+  uint32_t sum = 0;
+  for (uint32_t b = lo; b < hi; b++) {
+    const uint32_t st = base1[(uint64_t)b * nb1_total];
+    const uint32_t en = b + 1 < nbins1 ? base1[(uint64_t)(b + 1) * nb1_total] : tot;
+    seg[b] = st;
+    const uint32_t nt = (en - st + P2_TILE - 1) / P2_TILE;
+    ntiles[b] = nt;
+    sum += nt;
+  }
+  sh[t] = sum;
+  __syncthreads();
+  uint32_t v = sum;
+  for (uint32_t d = 1; d < 1024; d <<= 1) {
+    const uint32_t x = t >= d ? sh[t - d] : 0u;
+    __syncthreads();
+    v += x;
+    sh[t] = v;
+    __syncthreads();
+  }
+  uint32_t run = v - sum;
+  for (uint32_t b = lo; b < hi; b++) {
+    wbase[b] = run;
+    run += ntiles[b];
+  }
+  if (t == 1023) {
+    wbase[nbins1] = v;
+    seg[nbins1] = tot;
+  }
+}
+
+// pass-2 block -> (segment, tile, entry range); false past the last tile
+__device__ __forceinline__ bool p2_tile(const uint32_t* __restrict__ seg, const uint32_t* __restrict__ wbase,
+                                        uint32_t nbins1, uint32_t& sg, uint32_t& tile, uint32_t& e0,
+                                        uint32_t& e1) {
+  const uint32_t b = blockIdx.x;
+  if (b >= wbase[nbins1]) return false;
+  uint32_t lo = 0, hi = nbins1;  // largest sg with wbase[sg] <= b (and a tile: wbase[sg + 1] > b)
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (wbase[mid] <= b) lo = mid; else hi = mid;
+  }
+  sg = lo;
+  tile = b - wbase[lo];
+  e0 = seg[lo] + tile * P2_TILE;
+  e1 = min(e0 + P2_TILE, seg[lo + 1]);
+  return true;
+}
+
+__global__ __launch_bounds__(SORT_T) void k_sort_p2_count(const uint2* __restrict__ tmp,
+                                                          const uint32_t* __restrict__ seg,
+                                                          const uint32_t* __restrict__ ntiles,
+                                                          const uint32_t* __restrict__ wbase, uint32_t nbins1, int hb,
+                                                          uint32_t* __restrict__ hist2) {
+  __shared__ uint32_t h2[1u << P2_MAX_BITS];
+  uint32_t sg, tile, e0, e1;
+  if (!p2_tile(seg, wbase, nbins1, sg, tile, e0, e1)) return;
+  const uint32_t nbins2 = 1u << hb, mask = nbins2 - 1;
+  for (uint32_t b = threadIdx.x; b < nbins2; b += SORT_T) h2[b] = 0;
+  __syncthreads();
+  for (uint32_t e = e0 + threadIdx.x; e < e1; e += SORT_T) atomicAdd(&h2[tmp[e].x & mask], 1u);
+  __syncthreads();
+  const uint32_t nt = ntiles[sg];
+  const uint64_t base = (uint64_t)wbase[sg] * nbins2 + tile;
+  for (uint32_t b = threadIdx.x; b < nbins2; b += SORT_T) hist2[base + (uint64_t)b * nt] = h2[b];
+}
+
+__global__ __launch_bounds__(SORT_T) void k_sort_p2_scatter(const uint2* __restrict__ tmp,
+                                                            const uint32_t* __restrict__ seg,
+                                                            const uint32_t* __restrict__ ntiles,
+                                                            const uint32_t* __restrict__ wbase, uint32_t nbins1,
+                                                            int hb, const uint32_t* __restrict__ base2,
+                                                            uint2* __restrict__ sorted) {
+  __shared__ uint32_t c2[1u << P2_MAX_BITS];
+  uint32_t sg, tile, e0, e1;
+  if (!p2_tile(seg, wbase, nbins1, sg, tile, e0, e1)) return;
+  const uint32_t nbins2 = 1u << hb, mask = nbins2 - 1;
+  const uint32_t nt = ntiles[sg];
+  const uint64_t base = (uint64_t)wbase[sg] * nbins2 + tile;
+  for (uint32_t b = threadIdx.x; b < nbins2; b += SORT_T) c2[b] = base2[base + (uint64_t)b * nt];
+  __syncthreads();
+  for (uint32_t e = e0 + threadIdx.x; e < e1; e += SORT_T) {
+    const uint2 ent = tmp[e];
+    sorted[atomicAdd(&c2[ent.x & mask], 1u)] = ent;
+  }
+}
+
+// offsets[k] = first sorted position of key k (k <= nb; offsets[nb] = total)
+__global__ void k_sort_offsets(const uint32_t* __restrict__ seg, const uint32_t* __restrict__ ntiles,
+                               const uint32_t* __restrict__ wbase, const uint32_t* __restrict__ base2, uint32_t nb,
+                               int hb, uint32_t* __restrict__ offsets) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > nb) return;
+  const uint32_t sg = k >> hb, b = k & ((1u << hb) - 1u);
+  uint32_t v;
+  const uint32_t nbins1 = (nb + (1u << hb) - 1) >> hb;
+  if (k == nb || sg >= nbins1)
+    v = seg[nbins1];
+  else if (ntiles[sg] == 0)
+    v = seg[sg];
+  else
+    v = base2[(uint64_t)wbase[sg] * (1u << hb) + (uint64_t)b * ntiles[sg]];
+  offsets[k] = v;
+}
+
 __global__ void k_pack16(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint64_t nout) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nout) out[i] = (in[2 * i] & 0xFFFFu) | (in[2 * i + 1] << 16);
@@ -514,11 +717,12 @@ uint64_t msm_hist_words(const MsmPlan& p, const uint64_t* ns) {
   return std::max<uint64_t>(t, 1);
 }
 
-// digits + counting sort of (bucket, point) pairs by bucket
-void msm_sort(const uint32_t* const* d_scalars, const uint32_t* const* d_points, const uint64_t* ns,
-              const MsmPlan& p, hipStream_t s, int group, uint32_t* keys, uint32_t* vals, uint32_t* ranks,
-              uint32_t* counts, uint32_t* offsets, uint32_t* bsum, uint2* sorted, uint32_t* fix, uint64_t fix_words,
-              uint32_t* scal, uint32_t* hist) {
+// digits + single-pass counting sort of (bucket, point) pairs by bucket (the key spaces
+// above the two-pass sort's 2^22)
+static void msm_sort_single(const uint32_t* const* d_scalars, const uint32_t* const* d_points, const uint64_t* ns,
+                            const MsmPlan& p, hipStream_t s, int group, uint32_t* keys, uint32_t* vals,
+                            uint32_t* ranks, uint32_t* counts, uint32_t* offsets, uint32_t* bsum, uint2* sorted,
+                            uint32_t* fix, uint64_t fix_words, uint32_t* scal, uint32_t* hist) {
   uint32_t nblk = msm_scan_blocks(p.nb);
   const int fixed = p.fixed ? 1 : 0;
   {
@@ -569,6 +773,103 @@ void msm_sort(const uint32_t* const* d_scalars, const uint32_t* const* d_points,
       }
       off += (uint64_t)p.Wd * n;
     }
+    BG_HIP(hipGetLastError());
+  }
+}
+
+static void scan_excl(uint32_t* data, uint64_t n, uint32_t* bsum, uint32_t* total, hipStream_t s) {
+  const uint32_t nblk = msm_scan_blocks((uint32_t)n);
+  k_scan_local<<<nblk, SCAN_T, 0, s>>>(data, data, bsum, (uint32_t)n);
+  k_scan_top<<<1, SCAN_T, 0, s>>>(bsum, nblk, total);
+  k_scan_add<<<blocks_for(n, 256), 256, 0, s>>>(data, bsum, (uint32_t)n);
+}
+
+// Key bits of an MSM batch and the split between the two passes (hb low bits in pass 2);
+// false for the single-pass sort: below 2^21 entries (where its fewer dependent launches
+// win: 2^10 .. 2^14 proofs 4-12 % faster with it, 2^16 and up slower), or when the high
+// bits exceed the pass-1 histogram.  BG16_SORT=1 / 2 forces the single / two-pass sort.
+static bool two_pass_bits(const MsmPlan& p, int& hb, uint32_t& nbins1) {
+  const char* e = getenv("BG16_SORT");
+  if (e && e[0] == '1') return false;
+  if (!(e && e[0] == '2') && p.entries < (1ull << 21)) return false;
+  int kb = 0;
+  while ((1ull << kb) < (uint64_t)p.nb) kb++;
+  hb = std::min(kb, P2_MAX_BITS);
+  if (kb - hb > P1_MAX_BITS) return false;
+  nbins1 = (uint32_t)(((uint64_t)p.nb + (1ull << hb) - 1) >> hb);
+  return true;
+}
+
+void msm_sort(const uint32_t* const* d_scalars, const uint32_t* const* d_points, const uint64_t* ns,
+              const MsmPlan& p, hipStream_t s, int group, MsmWork& w, uint32_t* offsets, uint2* sorted,
+              uint32_t* fix, uint64_t fix_words) {
+  int hb = 0;
+  uint32_t nbins1 = 0;
+  if (!two_pass_bits(p, hb, nbins1)) {
+    uint32_t* counts = offsets - (p.nb + 1);
+    msm_sort_single(d_scalars, d_points, ns, p, s, group, w.keys.as<uint32_t>(p.entries),
+                    w.vals.as<uint32_t>(p.entries), w.ranks.as<uint32_t>(p.entries), counts, offsets,
+                    w.scan_tmp.as<uint32_t>((uint64_t)msm_scan_blocks(p.nb) + 8), sorted, fix, fix_words,
+                    w.scal.as<uint32_t>(std::max<uint64_t>(8 * p.n, 8)), w.hist.as<uint32_t>(msm_hist_words(p, ns)));
+    return;
+  }
+  uint32_t nb1 = 0;  // pass-1 blocks over the batch
+  for (uint32_t i = 0; i < p.M; i++) nb1 += (uint32_t)((ns[i] + P1_SCALARS - 1) / P1_SCALARS);
+  const uint64_t h1n = std::max<uint64_t>((uint64_t)nbins1 * nb1, 1);
+  const uint32_t g2 = (uint32_t)((p.entries + P2_TILE - 1) / P2_TILE) + nbins1;
+  const uint32_t nbins2 = 1u << hb;
+  const uint64_t h2n = (uint64_t)g2 * nbins2;
+  BG_REQUIRE(h1n < (1ull << 31) && h2n < (1ull << 31), BG_ERR_ARG, "msm: sort histograms too large");
+  uint32_t* scal = w.scal.as<uint32_t>(std::max<uint64_t>(8 * p.n, 8));
+  uint32_t* hist1 = w.hist.as<uint32_t>(h1n);
+  uint32_t* hist2 = w.ranks.as<uint32_t>(h2n);
+  uint2* tmp = reinterpret_cast<uint2*>(w.keys.as<uint32_t>(2 * std::max<uint64_t>(p.entries, 1)));
+  uint32_t* segw = w.vals.as<uint32_t>(3ull * (nbins1 + 1) + 8);
+  uint32_t* seg = segw;
+  uint32_t* ntiles = segw + nbins1 + 1;
+  uint32_t* wbase = ntiles + nbins1 + 1;
+  uint32_t* total = wbase + nbins1 + 1;
+  uint32_t* bsum = w.scan_tmp.as<uint32_t>((uint64_t)msm_scan_blocks((uint32_t)std::max(h1n, h2n)) + 8);
+  const int fixed = p.fixed ? 1 : 0;
+  const size_t lds1 = (size_t)nbins1 * 4;
+  {
+    ScopedKTime kt(group == 1 ? "msm_g1_digits" : "msm_g2_digits", s);
+    k_zero_u32<<<blocks_for(fix_words, 256), 256, 0, s>>>(fix, fix_words);
+    uint64_t soff = 0;
+    uint32_t blk = 0;
+    for (uint32_t i = 0; i < p.M; i++) {
+      const uint64_t n = ns[i];
+      if (n) {
+        k_prep_scalars<<<blocks_for(n, 256), 256, 0, s>>>(d_scalars[i], d_points[i], group == 1 ? 16 : 32,
+                                                           (uint32_t)n, scal + soff);
+        const SortArgs a{scal + soff, (uint32_t)n, i * p.kspan, blk, nb1, nbins1, p.c, p.Wd, fixed, hb};
+        k_sort_p1_count<<<(unsigned)((n + P1_SCALARS - 1) / P1_SCALARS), SORT_T, lds1, s>>>(a, hist1);
+        blk += (uint32_t)((n + P1_SCALARS - 1) / P1_SCALARS);
+      }
+      soff += 8 * n;
+    }
+    BG_HIP(hipGetLastError());
+  }
+  {
+    ScopedKTime kt(group == 1 ? "msm_g1_sort" : "msm_g2_sort", s);
+    scan_excl(hist1, h1n, bsum, total, s);
+    uint64_t soff = 0;
+    uint32_t blk = 0;
+    for (uint32_t i = 0; i < p.M; i++) {
+      const uint64_t n = ns[i];
+      if (n) {
+        const SortArgs a{scal + soff, (uint32_t)n, i * p.kspan, blk, nb1, nbins1, p.c, p.Wd, fixed, hb};
+        k_sort_p1_scatter<<<(unsigned)((n + P1_SCALARS - 1) / P1_SCALARS), SORT_T, lds1, s>>>(a, hist1, tmp);
+        blk += (uint32_t)((n + P1_SCALARS - 1) / P1_SCALARS);
+      }
+      soff += 8 * n;
+    }
+    k_sort_tiles<<<1, 1024, 0, s>>>(hist1, nb1, nbins1, total, seg, ntiles, wbase);
+    k_zero_u32<<<blocks_for(h2n, 256), 256, 0, s>>>(hist2, h2n);
+    k_sort_p2_count<<<g2, SORT_T, 0, s>>>(tmp, seg, ntiles, wbase, nbins1, hb, hist2);
+    scan_excl(hist2, h2n, bsum, total + 1, s);
+    k_sort_p2_scatter<<<g2, SORT_T, 0, s>>>(tmp, seg, ntiles, wbase, nbins1, hb, hist2, sorted);
+    k_sort_offsets<<<blocks_for((uint64_t)p.nb + 1, 256), 256, 0, s>>>(seg, ntiles, wbase, hist2, p.nb, hb, offsets);
     BG_HIP(hipGetLastError());
   }
 }

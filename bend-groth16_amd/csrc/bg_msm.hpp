@@ -111,10 +111,12 @@ void launch_unpack16(const uint32_t* in32, uint32_t* out16, uint64_t in_words, h
 // pipeline pieces shared by the per-group translation units
 uint32_t msm_scan_blocks(uint32_t n);
 void launch_zero_u32(uint32_t* p, uint64_t n, hipStream_t s);
+// digits and the sort of the batch's (bucket, point) entries by bucket: sorted entries,
+// offsets[0 .. nb] (offsets[nb] = entries); the work arrays come from w (offsets lies in
+// w.counts, after nb + 1 words the single-pass sort uses for its counts)
 void msm_sort(const uint32_t* const* d_scalars, const uint32_t* const* d_points, const uint64_t* ns,
-              const MsmPlan& p, hipStream_t s, int group, uint32_t* keys, uint32_t* vals, uint32_t* ranks,
-              uint32_t* counts, uint32_t* offsets, uint32_t* bsum, uint2* sorted, uint32_t* fix, uint64_t fix_words,
-              uint32_t* scal, uint32_t* hist);
+              const MsmPlan& p, hipStream_t s, int group, MsmWork& w, uint32_t* offsets, uint2* sorted,
+              uint32_t* fix, uint64_t fix_words);
 void enqueue_g1(uint32_t M, const uint32_t* const* d_points, const uint32_t* const* d_scalars, const uint64_t* ns,
                 int window_bits, hipStream_t s, MsmWork& w, bool fixed);
 void enqueue_g2(uint32_t M, const uint32_t* const* d_points, const uint32_t* const* d_scalars, const uint64_t* ns,

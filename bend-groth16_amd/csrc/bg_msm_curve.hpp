@@ -546,13 +546,8 @@ void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* 
   w.plan = p;
   w.group = group;
   const uint64_t xw = C::xyzz_words;
-  uint32_t* keys = w.keys.as<uint32_t>(p.entries);
-  uint32_t* vals = w.vals.as<uint32_t>(p.entries);
   uint32_t* counts = w.counts.as<uint32_t>(2 * ((uint64_t)p.nb + 1));  // counts | offsets
   uint32_t* offsets = counts + p.nb + 1;
-  uint32_t* ranks = w.ranks.as<uint32_t>(p.entries);
-  uint32_t nblk = msm_scan_blocks(p.nb);
-  uint32_t* bsum = w.scan_tmp.as<uint32_t>((uint64_t)nblk + 8);
   uint2* sorted = w.sorted.as<uint2>(p.entries);
   uint32_t* fix = w.fix.as<uint32_t>(1 + 2ull * p.nb);  // count | per-bucket flags | key list
   // reduction exception lists: level-1 count | groups, level-2 count | blocks
@@ -584,9 +579,7 @@ void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* 
     launch_zero_u32(outPQ, 2 * nout * xw, s);
   } else {
     BG_HIP(hipMemcpyAsync(dptrs, hptrs, 8ull * M, hipMemcpyHostToDevice, s));
-    msm_sort(d_scalars, d_points, ns, p, s, group, keys, vals, ranks, counts, offsets, bsum, sorted, fix,
-             1 + (uint64_t)p.nb, w.scal.as<uint32_t>(std::max<uint64_t>(8 * n, 8)),
-             w.hist.as<uint32_t>(msm_hist_words(p, ns)));
+    msm_sort(d_scalars, d_points, ns, p, s, group, w, offsets, sorted, fix, 1 + (uint64_t)p.nb);
     {
       {
         ScopedKTime kt(group == 1 ? "msm_g1_accumulate" : "msm_g2_accumulate", s);

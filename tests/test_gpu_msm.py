@@ -1,7 +1,10 @@
 """GPU parity of the Pippenger MSM (Msm.g1 / Msm.g2, bend2/src/msm.bend:293-298, :525-530)
 against the CPU oracle (brute-force-pinned Pippenger restatement) and, at large n, the
 closed form MSM = (sum a_i k_i mod r) G for bases P_i = k_i G.  Affine output is unique,
-so the comparison is bit-exact."""
+so the comparison is bit-exact.
+
+Every test runs with each bucket sort: the size rule (auto), the single-pass counting sort
+(BG16_SORT=1) and the two-pass sort (BG16_SORT=2) forced at every size."""
 import numpy as np
 import pytest
 
@@ -11,6 +14,13 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 R = O.P_R
+
+
+@pytest.fixture(autouse=True, params=["auto", "1", "2"], ids=["sort-auto", "sort-single", "sort-two-pass"])
+def sort_mode(request, monkeypatch):
+    if request.param != "auto":
+        monkeypatch.setenv("BG16_SORT", request.param)
+    return request.param
 
 
 def _u32(a):
