@@ -627,23 +627,61 @@ __global__ __launch_bounds__(SORT_T) void k_sort_p2_count(const uint2* __restric
   for (uint32_t b = threadIdx.x; b < nbins2; b += SORT_T) hist2[base + (uint64_t)b * nt] = h2[b];
 }
 
+// The tile is ordered by bin in LDS first (local histogram, scan, ranks), then written
+// out position by position: consecutive threads write consecutive slots of one bin's
+// range, so the stores coalesce (ranked in place, every entry would land at a random
+// 8-byte slot of its bin's range, about three times the bytes in partial lines).
 __global__ __launch_bounds__(SORT_T) void k_sort_p2_scatter(const uint2* __restrict__ tmp,
                                                             const uint32_t* __restrict__ seg,
                                                             const uint32_t* __restrict__ ntiles,
                                                             const uint32_t* __restrict__ wbase, uint32_t nbins1,
                                                             int hb, const uint32_t* __restrict__ base2,
                                                             uint2* __restrict__ sorted) {
-  __shared__ uint32_t c2[1u << P2_MAX_BITS];
+  __shared__ uint32_t lcount[1u << P2_MAX_BITS];  // counts, then local starts
+  __shared__ uint32_t lcur[1u << P2_MAX_BITS];
+  __shared__ uint32_t gbase[1u << P2_MAX_BITS];   // global start minus local start, per bin
+  __shared__ uint2 stage[P2_TILE];
   uint32_t sg, tile, e0, e1;
   if (!p2_tile(seg, wbase, nbins1, sg, tile, e0, e1)) return;
   const uint32_t nbins2 = 1u << hb, mask = nbins2 - 1;
   const uint32_t nt = ntiles[sg];
   const uint64_t base = (uint64_t)wbase[sg] * nbins2 + tile;
-  for (uint32_t b = threadIdx.x; b < nbins2; b += SORT_T) c2[b] = base2[base + (uint64_t)b * nt];
+  for (uint32_t b = threadIdx.x; b < nbins2; b += SORT_T) lcount[b] = 0;
   __syncthreads();
-  for (uint32_t e = e0 + threadIdx.x; e < e1; e += SORT_T) {
-    const uint2 ent = tmp[e];
-    sorted[atomicAdd(&c2[ent.x & mask], 1u)] = ent;
+  constexpr uint32_t PER = P2_TILE / SORT_T;
+  uint2 ent[PER];
+#pragma unroll
+  for (uint32_t k = 0; k < PER; k++) {
+    const uint32_t e = e0 + k * SORT_T + threadIdx.x;
+    ent[k] = e < e1 ? tmp[e] : make_uint2(0xFFFFFFFFu, 0u);
+    if (e < e1) atomicAdd(&lcount[ent[k].x & mask], 1u);
+  }
+  __syncthreads();
+  // exclusive scan of the (at most SORT_T) bin counts, one bin per thread
+  static_assert((1u << P2_MAX_BITS) <= SORT_T, "one thread per pass-2 bin");
+  const uint32_t b = threadIdx.x;
+  const uint32_t own = b < nbins2 ? lcount[b] : 0u;
+  uint32_t incl = own;
+  for (uint32_t d = 1; d < nbins2; d <<= 1) {
+    const uint32_t x = b >= d && b < nbins2 ? lcount[b - d] : 0u;
+    __syncthreads();
+    incl += x;
+    if (b < nbins2) lcount[b] = incl;
+    __syncthreads();
+  }
+  if (b < nbins2) {
+    lcur[b] = incl - own;
+    gbase[b] = base2[base + (uint64_t)b * nt] - (incl - own);
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t k = 0; k < PER; k++)
+    if (ent[k].x != 0xFFFFFFFFu) stage[atomicAdd(&lcur[ent[k].x & mask], 1u)] = ent[k];
+  __syncthreads();
+  const uint32_t cnt = e1 - e0;
+  for (uint32_t q = threadIdx.x; q < cnt; q += SORT_T) {
+    const uint2 v = stage[q];
+    sorted[gbase[v.x & mask] + q] = v;
   }
 }
 
