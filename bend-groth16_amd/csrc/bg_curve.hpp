@@ -91,25 +91,6 @@ struct Curve {
     return {x3, y3, F::mul(p.ZZ, pp), F::mul(p.ZZZ, ppp)};
   }
 
-  // p + a by the generic madd-2008-s formula only, for p not infinity and a not
-  // infinity; sets bad when x(p) = x(a) (doubling or cancellation), where the result is
-  // not the sum and the caller must recompute exactly.
-  __device__ __forceinline__ static P madd_generic(const P& p, const A& a, uint32_t& bad) {
-    E u2 = F::mul(a.x, p.ZZ);
-    E pp_ = F::sub(u2, p.X);
-    bad |= F::is_zero(pp_) ? 1u : 0u;
-    E s2 = F::mul(a.y, p.ZZZ);
-    E r = F::sub(s2, p.Y);
-    E pp = F::sqr(pp_);
-    E zz3 = F::mul(p.ZZ, pp);
-    E ppp = F::mul(pp_, pp);
-    E zzz3 = F::mul(p.ZZZ, ppp);
-    E q = F::mul(p.X, pp);
-    E x3 = F::sub(F::sub(F::sqr(r), ppp), F::dbl(q));
-    E y3 = F::mul_sub(r, F::sub(q, x3), p.Y, ppp);
-    return {x3, y3, zz3, zzz3};
-  }
-
   // p + q by the generic add-2008-s formula, for p and q not infinity; sets bad when
   // x(p) = x(q), where the result is not the sum.
   __device__ __forceinline__ static P add_generic(const P& p, const P& q, uint32_t& bad) {

@@ -33,45 +33,8 @@ __device__ __forceinline__ const uint32_t* entry_points(const uint64_t* __restri
   return reinterpret_cast<const uint32_t*>(ptrs[key / kspan]);
 }
 
-// The running sum of a G1 run lives in the digit domain (bg_curve_fd.hpp); points are
-// read and buckets written in the word domain.  G2 keeps the word-domain formula: its
-// digit-domain state (four Fq2 coordinates of 18 digits plus the operands of the
-// one-reduction Fq2 products) does not fit two waves' registers per SIMD without spills.
-template <class F>
-struct WordAcc {
-  using C = Curve<F>;
-  using Pt = typename C::P;
-  using Raw = typename C::A;
-  static constexpr int aff_words = C::aff_words, xyzz_words = C::xyzz_words;
-  static constexpr int ew = C::elem_words;
-  __device__ __forceinline__ static typename F::E elem(const uint32_t* w) {
-    typename F::E e;
-    uint32_t* d = reinterpret_cast<uint32_t*>(&e);
-#pragma unroll
-    for (int i = 0; i < ew; i++) d[i] = w[i];
-    return e;
-  }
-  __device__ __forceinline__ static void put(uint32_t* w, const typename F::E& e) {
-    const uint32_t* d = reinterpret_cast<const uint32_t*>(&e);
-#pragma unroll
-    for (int i = 0; i < ew; i++) w[i] = d[i];
-  }
-  __device__ __forceinline__ static Raw load(const uint32_t* w) { return {elem(w), elem(w + ew)}; }
-  __device__ __forceinline__ static Pt start(const Raw& a, bool neg) {
-    return {a.x, neg ? F::neg(a.y) : a.y, F::one(), F::one()};
-  }
-  __device__ __forceinline__ static Pt madd(const Pt& p, Raw a, bool neg, uint32_t& bad) {
-    if (neg) a.y = F::neg(a.y);
-    return C::madd_generic(p, a, bad);
-  }
-  __device__ __forceinline__ static void store(uint32_t* w, const Pt& p) {
-    put(w, p.X);
-    put(w + ew, p.Y);
-    put(w + 2 * ew, p.ZZ);
-    put(w + 3 * ew, p.ZZZ);
-  }
-};
-
+// The running sum of a run lives in the digit domain (bg_curve_fd.hpp); points are read
+// and buckets written in the word domain.
 template <class F>
 struct AccOf;
 template <>
@@ -80,7 +43,7 @@ struct AccOf<Field<FqP>> {
 };
 template <>
 struct AccOf<Fq2> {
-  using type = WordAcc<Fq2>;
+  using type = fd::G2Acc<fd::FqDig>;
 };
 
 constexpr bool same_digits(const uint32_t* a, const uint32_t* b) {
