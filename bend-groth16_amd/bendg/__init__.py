@@ -68,6 +68,7 @@ def lib() -> ctypes.CDLL:
         "bg_pk_header": [P, P],
         "bg_pk_device_bytes": [P, P],
         "bg_prove": [P, P, P, P, I, P],
+        "bg_graph_info": [P],
         "bg_proof_format": [P, ctypes.c_char_p, I],
         "bg_setup_square_chain": [ctypes.c_char_p, I, U64, I],
         "bg_kat_field_mul_xor": [I, P],
@@ -127,6 +128,13 @@ def _u32(a) -> np.ndarray:
 
 def init(device: int = 0) -> None:
     _check(lib().bg_init(device))
+
+
+def graph_info() -> dict:
+    """Process-wide counts of the single-proof HIP graphs (``bg_graph_info``)."""
+    v = np.zeros(3, dtype=np.uint64)
+    _check(lib().bg_graph_info(_p(v)))
+    return {"recorded": int(v[0]), "launches": int(v[1]), "failed": int(v[2])}
 
 
 # ---------------------------------------------------------------- encodings

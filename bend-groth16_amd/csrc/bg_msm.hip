@@ -1,5 +1,6 @@
 // Pippenger MSM on gfx950 (C ABI: bg_msm_g1, bg_msm_g2).  See bg_msm.hpp for the shape.
 #include "bg_msm.hpp"
+#include <chrono>
 #include "bg_curve.hpp"
 #include "bg_host_ec.hpp"
 #include "bg_tune.hpp"
@@ -737,6 +738,49 @@ void launch_zero_u32(uint32_t* p, uint64_t n, hipStream_t s) {
   BG_HIP(hipGetLastError());
 }
 
+// Up to SET_PTRS_MAX base-array addresses passed by value, so a recorded launch sequence
+// carries them itself (a copy from a host staging buffer would read whatever the buffer
+// holds when the sequence is replayed).
+static constexpr uint32_t SET_PTRS_MAX = 8;
+struct PtrTable {
+  uint64_t p[SET_PTRS_MAX];
+};
+__global__ void k_set_ptrs(uint64_t* __restrict__ out, PtrTable t, uint32_t M) {
+  if (threadIdx.x < M) out[threadIdx.x] = t.p[threadIdx.x];
+}
+
+void launch_set_ptrs(uint64_t* d_out, const uint64_t* h_ptrs, uint32_t M, hipStream_t s) {
+  if (M <= SET_PTRS_MAX) {
+    PtrTable t{};
+    for (uint32_t i = 0; i < M; i++) t.p[i] = h_ptrs[i];
+    k_set_ptrs<<<1, 64, 0, s>>>(d_out, t, M);
+    BG_HIP(hipGetLastError());
+  } else {
+    BG_REQUIRE(!stream_capturing(s), BG_ERR_INTERNAL, "msm: a captured batch holds at most %u base arrays",
+               SET_PTRS_MAX);
+    BG_HIP(hipMemcpyAsync(d_out, h_ptrs, 8ull * M, hipMemcpyHostToDevice, s));
+  }
+}
+
+__global__ void k_publish(const uint32_t* __restrict__ src, uint64_t n, uint32_t* __restrict__ dst) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+// Runs after k_publish has completed (stream order), so its stores have reached the host.
+__global__ void k_raise_flag(uint32_t* flag) {
+  if (threadIdx.x == 0) {
+    __threadfence_system();
+    __hip_atomic_store(flag, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+void launch_publish(const uint32_t* src, uint64_t n, uint32_t* dst, uint32_t* flag, hipStream_t s) {
+  if (n) k_publish<<<(unsigned)std::min<uint64_t>(blocks_for(n, 256), 64), 256, 0, s>>>(src, n, dst);
+  k_raise_flag<<<1, 64, 0, s>>>(flag);
+  BG_HIP(hipGetLastError());
+}
+
 // chunk of scalars per counting block: 2^14, or 2^15 when n/32 or half the bucket count
 // is larger (the 16-bit LDS counters bound it at 2^15)
 static int count_chunk_log2(uint64_t n, uint32_t B) {
@@ -1002,9 +1046,36 @@ static void host_finish(const MsmPlan& p, const uint32_t* hout, std::vector<bgh:
     host_parallel((int)p.M, horner);
 }
 
+// Waits for the flag of a recorded MSM's outputs (msm_enqueue under capture); the graph's
+// stream is polled so that a failed launch ends the wait with its error.
+static void wait_published(const MsmWork& w) {
+  const uint32_t* flag = static_cast<const uint32_t*>(w.host_pub.ptr);
+  auto last = std::chrono::steady_clock::now();
+  while (__atomic_load_n(flag, __ATOMIC_ACQUIRE) == 0) {
+    __builtin_ia32_pause();
+    // the stream is asked only every 200 us: the runtime's queries take its locks
+    const auto now = std::chrono::steady_clock::now();
+    if (now - last < std::chrono::microseconds(200)) continue;
+    last = now;
+    const hipError_t q = hipStreamQuery(w.pub_stream);
+    if (q == hipSuccess) {
+      BG_REQUIRE(__atomic_load_n(flag, __ATOMIC_ACQUIRE) != 0, BG_ERR_INTERNAL,
+                 "msm: a recorded launch finished without publishing its outputs");
+      break;
+    }
+    if (q != hipErrorNotReady) BG_HIP(q);
+  }
+}
+
 void msm_finish(MsmWork& w, uint32_t* out_affine, uint32_t* xyzz_out) {
-  BG_HIP(hipEventSynchronize(w.done));
-  const uint32_t* hout = static_cast<const uint32_t*>(w.host_out.ptr);
+  const uint32_t* hout;
+  if (w.published) {
+    wait_published(w);
+    hout = static_cast<const uint32_t*>(w.host_pub.ptr) + MsmWork::PUB_HEAD;
+  } else {
+    BG_HIP(hipEventSynchronize(w.done));
+    hout = static_cast<const uint32_t*>(w.host_out.ptr);
+  }
   if (w.group == 1) {
     std::vector<bgh::P1> r;
     host_finish<bgh::Fq>(w.plan, hout, r);

@@ -49,6 +49,18 @@ struct MsmWork {
   DevBuf keys, vals, ranks, counts, sorted, buckets, pkeys, pflags, ppts, mkeys, mflags, mpts, lvlS, lvlT, outPQ,
       scan_tmp, fix, rfix, ptrs, scal, hist, mq;
   HostBuf host_out, host_ptrs;
+  // Coherent pinned landing of the outputs of a recorded MSM (no event nodes in a graph):
+  // PUB_HEAD words whose first is a flag the device raises once the outputs follow it.
+  static constexpr uint64_t PUB_HEAD = 64;
+  HostBuf host_pub;
+  bool published = false;        // outputs come through host_pub (set for graph launches)
+  hipStream_t pub_stream = nullptr;  // stream the recorded sequence runs on (polled for errors)
+  // Clears the flag before the recorded sequence that holds this MSM is launched on s.
+  void arm_publish(hipStream_t s) {
+    published = true;
+    pub_stream = s;
+    __atomic_store_n(static_cast<uint32_t*>(host_pub.ptr), 0u, __ATOMIC_RELEASE);
+  }
   MsmPlan plan;
   int group = 1;
   hipEvent_t done = nullptr;
@@ -66,6 +78,8 @@ struct MsmWork {
       b->release();
     host_out.release();
     host_ptrs.release();
+    host_pub.release();
+    published = false;
     if (done) (void)hipEventDestroy(done);
     if (mid) (void)hipEventDestroy(mid);
     done = mid = nullptr;
@@ -111,6 +125,12 @@ void launch_unpack16(const uint32_t* in32, uint32_t* out16, uint64_t in_words, h
 // pipeline pieces shared by the per-group translation units
 uint32_t msm_scan_blocks(uint32_t n);
 void launch_zero_u32(uint32_t* p, uint64_t n, hipStream_t s);
+// d_out[i] = h_ptrs[i] for i < M on stream s (the MSM's table of base-array addresses;
+// h_ptrs may be reused once this returns).
+void launch_set_ptrs(uint64_t* d_out, const uint64_t* h_ptrs, uint32_t M, hipStream_t s);
+// dst[i] = src[i] for i < n (dst in coherent pinned host memory), then *flag = 1 once the
+// copy is complete and visible to the host.
+void launch_publish(const uint32_t* src, uint64_t n, uint32_t* dst, uint32_t* flag, hipStream_t s);
 // digits and the sort of the batch's (bucket, point) entries by bucket: sorted entries,
 // offsets[0 .. nb] (offsets[nb] = entries); the work arrays come from w (offsets lies in
 // w.counts, after nb + 1 words the single-pass sort uses for its counts)

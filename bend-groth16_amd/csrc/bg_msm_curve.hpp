@@ -541,7 +541,7 @@ void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* 
     // empty MSM: every bucket empty -> zero outputs
     launch_zero_u32(outPQ, 2 * nout * xw, s);
   } else {
-    BG_HIP(hipMemcpyAsync(dptrs, hptrs, 8ull * M, hipMemcpyHostToDevice, s));
+    launch_set_ptrs(dptrs, hptrs, M, s);
     msm_sort(d_scalars, d_points, ns, p, s, group, w, offsets, sorted, fix, 1 + (uint64_t)p.nb);
     {
       {
@@ -576,7 +576,7 @@ void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* 
       BG_HIP(hipGetLastError());
     }
     if (!w.mid) BG_HIP(hipEventCreateWithFlags(&w.mid, hipEventDisableTiming));
-    BG_HIP(hipEventRecord(w.mid, s));
+    record_if_eager(w.mid, s);
     {
       int log2m = 0;
       while ((1u << log2m) < p.m) log2m++;
@@ -607,9 +607,21 @@ void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* 
       BG_HIP(hipGetLastError());
     }
   }
-  BG_HIP(hipMemcpyAsync(hout, outPQ, 2 * nout * xw * 4, hipMemcpyDeviceToHost, s));
-  if (!w.done) BG_HIP(hipEventCreateWithFlags(&w.done, hipEventDisableTiming));
-  BG_HIP(hipEventRecord(w.done, s));
+  // Outputs to the host: eagerly a copy and an event; in a recorded graph (which carries
+  // no event nodes) a kernel writes them to coherent pinned memory and raises a flag that
+  // msm_finish polls.  The landing is sized on eager runs too, so recording allocates
+  // nothing.
+  const uint64_t out_words = 2 * nout * xw;
+  uint32_t* pub = static_cast<uint32_t*>(w.host_pub.get(4 * (MsmWork::PUB_HEAD + out_words), hipHostMallocCoherent));
+  if (stream_capturing(s)) {
+    launch_publish(outPQ, out_words, pub + MsmWork::PUB_HEAD, pub, s);
+    w.published = true;
+  } else {
+    BG_HIP(hipMemcpyAsync(hout, outPQ, out_words * 4, hipMemcpyDeviceToHost, s));
+    if (!w.done) BG_HIP(hipEventCreateWithFlags(&w.done, hipEventDisableTiming));
+    BG_HIP(hipEventRecord(w.done, s));
+    w.published = false;
+  }
 }
 
 

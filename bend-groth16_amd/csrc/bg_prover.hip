@@ -10,6 +10,7 @@
 #include "bg_runtime.hpp"
 #include "bg_msm.hpp"
 #include "bg_prover.hpp"
+#include "bg_tune.hpp"
 #include "bg_ntt.hpp"
 #include "bg_field.hpp"
 #include "bg_host_ec.hpp"
@@ -19,6 +20,7 @@
 #include <fstream>
 #include <string>
 #include <vector>
+#include <unistd.h>
 
 namespace bg {
 
@@ -140,6 +142,9 @@ struct bg_pk {
   std::mutex own_mu;
   // proofs of this key queued in an engine or running (bg_pk_free refuses while > 0)
   std::atomic<int> inflight{0};
+  // process-unique id (recorded launch sequences are keyed by it, not by the address,
+  // which a later key may reuse)
+  uint64_t serial = 0;
 };
 
 namespace bg {
@@ -264,6 +269,8 @@ extern "C" int bg_pk_load(const char* dir, bg_pk** out) {
     // 1. read and validate every file on the host (bend2/tools/verify_bin.py:24-64 checks)
     auto hdrw = read_words(d + "/header.bin", 8);
     auto pk = std::make_unique<bg_pk>();
+    static std::atomic<uint64_t> next_serial{1};
+    pk->serial = next_serial.fetch_add(1);
     memcpy(pk->hdr, hdrw.data(), 32);
     uint32_t nc = pk->hdr[0], ni = pk->hdr[1], nw = pk->hdr[2], lg = pk->hdr[3];
     uint32_t na = pk->hdr[4], nbq = pk->hdr[5], nh = pk->hdr[6], nl = pk->hdr[7];
@@ -389,14 +396,93 @@ ProveSlot::ProveSlot() {
   (void)ctx();  // device current on this thread
   for (auto& x : s) BG_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
   for (auto& e : ev) BG_HIP(hipEventCreate(&e));
+  for (auto& e : join) BG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 }
 
 ProveSlot::~ProveSlot() {
   release_buffers();
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : join)
+    if (e) (void)hipEventDestroy(e);
   for (auto& x : s)
     if (x) (void)hipStreamDestroy(x);
+}
+
+const ProveSlot::Graph* ProveSlot::find_graph(uint64_t key) {
+  for (auto& g : graphs)
+    if (g.key == key) {
+      g.used = ++graph_clock;
+      for (int i = 0; i < 3; i++) {
+        work[i].plan = g.plan[i];
+        work[i].group = g.group[i];
+      }
+      return &g;
+    }
+  return nullptr;
+}
+
+static void destroy_graph(ProveSlot::Graph& g) {
+  for (auto& e : g.exec)
+    if (e) (void)hipGraphExecDestroy(e);
+}
+
+const ProveSlot::Graph* ProveSlot::add_graph(uint64_t key, const hipGraphExec_t exec[3]) {
+  if (graphs.size() >= MAX_GRAPHS) {
+    auto lru = graphs.begin();
+    for (auto it = graphs.begin(); it != graphs.end(); ++it)
+      if (it->used < lru->used) lru = it;
+    destroy_graph(*lru);
+    graphs.erase(lru);
+  }
+  Graph g;
+  g.key = key;
+  for (int i = 0; i < 3; i++) g.exec[i] = exec[i];
+  g.used = ++graph_clock;
+  for (int i = 0; i < 3; i++) {
+    g.plan[i] = work[i].plan;
+    g.group[i] = work[i].group;
+  }
+  graphs.push_back(g);
+  return &graphs.back();
+}
+
+void ProveSlot::drop_graphs() {
+  for (auto& g : graphs) destroy_graph(g);
+  graphs.clear();
+  warm.clear();
+  failed_key = 0;
+}
+
+bool ProveSlot::is_warm(uint64_t key) const {
+  for (uint64_t k : warm)
+    if (k == key) return true;
+  return false;
+}
+
+void ProveSlot::add_warm(uint64_t key) {
+  if (is_warm(key)) return;
+  if (warm.size() >= 2 * MAX_GRAPHS) warm.erase(warm.begin());
+  warm.push_back(key);
+}
+
+static uint64_t mix64(uint64_t h, uint64_t v) {
+  h ^= v + 0x9e3779b97f4a7c15ULL + (h << 6) + (h >> 2);
+  return h * 0xff51afd7ed558ccdULL;
+}
+
+uint64_t ProveSlot::buffer_signature() const {
+  uint64_t sig = 0x6a09e667f3bcc909ULL;
+  for (const DevArray* a : {&z, &zc, &ea, &eb, &ec, &h, &tmp}) sig = mix64(mix64(sig, (uint64_t)a->ptr), a->bytes);
+  for (const MsmWork& w : work) {
+    for (const DevBuf* b : {&w.keys, &w.vals, &w.ranks, &w.counts, &w.sorted, &w.buckets, &w.pkeys, &w.pflags,
+                            &w.ppts, &w.mkeys, &w.mflags, &w.mpts, &w.lvlS, &w.lvlT, &w.outPQ, &w.scan_tmp, &w.fix,
+                            &w.rfix, &w.ptrs, &w.scal, &w.hist, &w.mq})
+      sig = mix64(mix64(sig, (uint64_t)b->ptr), b->cap);
+    for (const HostBuf* b : {&w.host_out, &w.host_ptrs, &w.host_pub}) sig = mix64(mix64(sig, (uint64_t)b->ptr), b->cap);
+    sig = mix64(mix64(sig, (uint64_t)w.done), (uint64_t)w.mid);
+  }
+  return sig;
 }
 
 void ProveSlot::ensure(uint64_t nz, uint64_t d) {
@@ -412,6 +498,7 @@ void ProveSlot::ensure(uint64_t nz, uint64_t d) {
 }
 
 void ProveSlot::release_buffers() {
+  drop_graphs();
   for (DevArray* a : {&z, &zc, &ea, &eb, &ec, &h, &tmp, &comb}) a->release();
   zpin.release();
   proofpin.release();
@@ -533,6 +620,72 @@ void check_host_inputs(const uint32_t* witness, const uint32_t* rs, int flags, u
   }
 }
 
+// Single proofs replay recorded HIP graphs of their device work (BG16_GRAPH: 0 never,
+// 1 always; unset: domains up to 2^GRAPH_DEFAULT_MAX_LOG2).  Measured per proof, eager vs
+// replayed: K=10 1.005 / 0.989 ms, K=12 1.117 / 1.094, K=14 1.340 / 1.296, K=16 1.856 /
+// 1.925, K=18 4.066 / 4.188 -- small proofs gain the host's launch work, large ones lose
+// a little to the graphs' own dispatch.
+static constexpr uint32_t GRAPH_DEFAULT_MAX_LOG2 = 15;
+static bool graph_wanted(uint32_t lg) {
+  const char* e = getenv("BG16_GRAPH");
+  if (e && e[0]) return atoi(e) != 0;
+  return lg <= GRAPH_DEFAULT_MAX_LOG2;
+}
+
+// Everything a recorded proof depends on besides its inputs: the key, the schedule, the
+// tuning table, every BG16_ setting (window, slice and sort overrides change the launch
+// sequence) and the addresses and sizes of the slot's buffers.
+static uint64_t graph_key(const bg_pk* pk, const ProveSlot& sl, int sched) {
+  uint64_t k = mix64(mix64(mix64(0x243f6a8885a308d3ULL, pk->serial), (uint64_t)sched), tune_generation());
+  for (char** e = environ; e && *e; e++)
+    if (strncmp(*e, "BG16_", 5) == 0)
+      for (const char* c = *e; *c; c++) k = mix64(k, (uint64_t)(unsigned char)*c);
+  return mix64(k, sl.buffer_signature());
+}
+
+// Records fn's launches on the slot's streams as one graph (s[0] forks the side streams;
+// they are joined back before the recording ends) and instantiates it.  Nothing runs
+// while recording.  Returns null, with the streams usable, if the recording fails; the
+// proof then runs eagerly (and reports any real error from there).
+static std::atomic<uint64_t> g_graphs_recorded{0}, g_graph_launches{0}, g_graph_failures{0};
+
+// Records fn's launches on stream st as a linear graph and instantiates it (nothing runs
+// while recording); null if recording fails, with st usable again.
+template <class Fn>
+static hipGraphExec_t record_linear(hipStream_t st, Fn&& fn) {
+  if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+  hipGraph_t g = nullptr;
+  bool ok = true;
+  try {
+    fn();
+  } catch (const Error&) {
+    ok = false;
+  }
+  if (hipStreamEndCapture(st, &g) != hipSuccess) ok = false;
+  hipGraphExec_t exec = nullptr;
+  if (ok && g && hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess) exec = nullptr;
+  if (g) (void)hipGraphDestroy(g);
+  return ok ? exec : nullptr;
+}
+
+// Records schedule 0's three per-stream sequences (B2 on s[3], the G1 batch on s[1], the
+// rest on s[0]) for key.  On failure nothing is kept, the key is not tried again on this
+// slot, and the proof runs eagerly (which reports any real error).
+template <class F3, class F1, class F0>
+static const ProveSlot::Graph* record_graphs(ProveSlot& sl, uint64_t key, F3&& on_s3, F1&& on_s1, F0&& on_s0) {
+  hipGraphExec_t ex[3] = {record_linear(sl.s[3], on_s3), record_linear(sl.s[1], on_s1), record_linear(sl.s[0], on_s0)};
+  (void)hipGetLastError();
+  if (!ex[0] || !ex[1] || !ex[2]) {
+    for (auto e : ex)
+      if (e) (void)hipGraphExecDestroy(e);
+    sl.failed_key = key;
+    g_graph_failures.fetch_add(1);
+    return nullptr;
+  }
+  g_graphs_recorded.fetch_add(1);
+  return sl.add_graph(key, ex);
+}
+
 void prove_on(bg_pk* pk, ProveSlot& sl, const uint32_t* witness, const uint32_t* rs, uint32_t* proof_out, int flags,
               double* t_ms) {
   {
@@ -593,23 +746,15 @@ void prove_on(bg_pk* pk, ProveSlot& sl, const uint32_t* witness, const uint32_t*
     uint32_t* z = sl.z.as<uint32_t>();
     upload_witness(witness, flags, nz, z, static_cast<uint32_t*>(sl.zpin.get(32 * nz)), s0);
     BG_HIP(hipEventRecord(ev[1], s0));
-    // 2. evaluation vectors and canonical scalars
-    uint32_t* zc = sl.zc.as<uint32_t>();
-    {
-      ScopedKTime kt("evals", s0);
-      k_from_mont<<<nb(nz, 256), 256, 0, s0>>>(z, zc, nz);
-      BG_HIP(hipEventRecord(ev[2], s0));  // canonical scalars ready
-      k_evals<<<nb(d, 256), 256, 0, s0>>>(pk->A.offs.as<uint32_t>(), pk->A.cols.as<uint32_t>(),
-                                          pk->A.coef.as<uint32_t>(), z, nc, ni, 1, d, sl.ea.as<uint32_t>());
-      k_evals<<<nb(d, 256), 256, 0, s0>>>(pk->B.offs.as<uint32_t>(), pk->B.cols.as<uint32_t>(),
-                                          pk->B.coef.as<uint32_t>(), z, nc, ni, 0, d, sl.eb.as<uint32_t>());
-      k_evals<<<nb(d, 256), 256, 0, s0>>>(pk->C.offs.as<uint32_t>(), pk->C.cols.as<uint32_t>(),
-                                          pk->C.coef.as<uint32_t>(), z, nc, ni, 0, d, sl.ec.as<uint32_t>());
-      BG_HIP(hipGetLastError());
-    }
-    BG_HIP(hipEventRecord(ev[3], s0));
-    // 3. qap_h and the MSMs.  The G1 MSMs other than H run as one batch (one sort,
-    //    accumulation and reduction over all their windows); H needs qap_h's output.
+    // 2.-4. the device work of the proof after the upload: evaluation vectors, canonical
+    //    scalars, qap_h and the five MSMs, each MSM's result copied to pinned host memory
+    //    and its event recorded.  Run eagerly, or recorded once as a HIP graph and replayed
+    //    (one launch instead of ~100 kernel launches and the events between streams).
+    // 2.-4. The device work after the upload.  `head` makes the canonical scalars and forks
+    //    the side streams from them; then each stream has its own launch sequence: B2 on s3,
+    //    the G1 batch A, B1, L on s1, and the evaluation vectors, qap_h and H on s0.  The
+    //    G1 MSMs other than H run as one batch (one sort, accumulation and reduction over
+    //    all their windows); H needs qap_h's output.
     //    Schedules (BG16_PROVE_SCHEDULE; every one gives the same proof):
     //   0: B2 on s3 and the G1 batch A, B1, L on s1 start at once; qap_h then H on s0
     //      (default: the shortest from 2^12 to 2^18)
@@ -619,19 +764,44 @@ void prove_on(bg_pk* pk, ProveSlot& sl, const uint32_t* witness, const uint32_t*
     //   4: qap_h first; then B2 on s3, the G1 batch A, B1, L on s1 and H on s0
     //   5: B2, qap_h, the G1 batch and H on a single stream (no overlap: per-kernel times
     //      for profiling)
-    for (hipStream_t sx : {s1, s2, s3}) BG_HIP(hipStreamWaitEvent(sx, ev[2], 0));
+    //    Schedule 0's three per-stream sequences are recorded as linear HIP graphs on the
+    //    second proof of a key on this slot and replayed after that (one launch per stream
+    //    instead of ~100 kernel launches).  A graph that forks and joins streams itself
+    //    was measured slower: this runtime's launch of it waits on the host for its
+    //    branches (~1.2 ms per launch of a 2^10 proof).
+    uint32_t* zc = sl.zc.as<uint32_t>();
     const uint32_t* g1p[4] = {pk->pa.as<uint32_t>(), pk->pb1.as<uint32_t>(), pk->pl.as<uint32_t>(),
                               pk->ph.as<uint32_t>()};
     const uint32_t* g1s[4] = {zc, zc, zc + 8 * ni, sl.h.as<uint32_t>()};
     const uint64_t g1n[4] = {na, nbq, nl, nh};
+    auto head = [&] {
+      ScopedKTime kt("evals", s0);
+      k_from_mont<<<nb(nz, 256), 256, 0, s0>>>(z, zc, nz);
+      BG_HIP(hipGetLastError());
+      BG_HIP(hipEventRecord(ev[2], s0));  // canonical scalars ready
+      for (hipStream_t sx : {s1, s2, s3}) BG_HIP(hipStreamWaitEvent(sx, ev[2], 0));
+    };
+    auto run_evals = [&] {
+      {
+        ScopedKTime kt("evals", s0);
+        k_evals<<<nb(d, 256), 256, 0, s0>>>(pk->A.offs.as<uint32_t>(), pk->A.cols.as<uint32_t>(),
+                                            pk->A.coef.as<uint32_t>(), z, nc, ni, 1, d, sl.ea.as<uint32_t>());
+        k_evals<<<nb(d, 256), 256, 0, s0>>>(pk->B.offs.as<uint32_t>(), pk->B.cols.as<uint32_t>(),
+                                            pk->B.coef.as<uint32_t>(), z, nc, ni, 0, d, sl.eb.as<uint32_t>());
+        k_evals<<<nb(d, 256), 256, 0, s0>>>(pk->C.offs.as<uint32_t>(), pk->C.cols.as<uint32_t>(),
+                                            pk->C.coef.as<uint32_t>(), z, nc, ni, 0, d, sl.ec.as<uint32_t>());
+        BG_HIP(hipGetLastError());
+      }
+      record_if_eager(ev[3], s0);
+    };
     auto run_qap = [&] {
       ScopedKTime kt("qap_h", s0);
       qap_h_run(sl.ea.as<uint32_t>(), sl.eb.as<uint32_t>(), sl.ec.as<uint32_t>(), sl.h.as<uint32_t>(),
                 sl.tmp.as<uint32_t>(), (int)lg, true, s0);
-      BG_HIP(hipEventRecord(ev[4], s0));
+      record_if_eager(ev[4], s0);
     };
     auto run_b2 = [&] {
-      BG_HIP(hipEventRecord(ev[9], s3));
+      record_if_eager(ev[9], s3);
       if (pk->fixed) {
         const uint32_t* tb = pk->pb2.as<uint32_t>();
         const uint64_t nn = nbq;
@@ -639,35 +809,17 @@ void prove_on(bg_pk* pk, ProveSlot& sl, const uint32_t* witness, const uint32_t*
       } else {
         msm_enqueue(2, pk->pb2.as<uint32_t>(), zc, nbq, 0, s3, sl.work[2]);
       }
-      BG_HIP(hipEventRecord(ev[10], s3));
+      record_if_eager(ev[10], s3);
     };
     auto run_g1_batch3 = [&] {
-      BG_HIP(hipEventRecord(ev[5], s1));
+      record_if_eager(ev[5], s1);
       if (pk->fixed)
         msm_enqueue_fixed(1, 3, g1p, g1s, g1n, pk->c_g1, s1, sl.work[0]);
       else
         msm_enqueue_batch(1, 3, g1p, g1s, g1n, 0, s1, sl.work[0]);
-      BG_HIP(hipEventRecord(ev[6], s1));
+      record_if_eager(ev[6], s1);
     };
-    if (sched == 4) {
-      run_qap();
-      BG_HIP(hipStreamWaitEvent(s3, ev[4], 0));
-      BG_HIP(hipStreamWaitEvent(s1, ev[4], 0));
-      run_b2();
-      run_g1_batch3();
-    } else {
-      run_b2();
-      if (sched == 0 || sched == 2) {
-        if (sched == 2) BG_HIP(hipStreamWaitEvent(s1, sl.work[2].mid, 0));
-        run_g1_batch3();
-      }
-      run_qap();
-      if (sched == 3 || sched == 5) {
-        BG_HIP(hipStreamWaitEvent(s1, ev[4], 0));
-        run_g1_batch3();
-      }
-    }
-    if (sched != 1) {
+    auto run_h = [&] {
       if (pk->fixed) {
         const uint32_t* tb = pk->ph.as<uint32_t>();
         const uint32_t* hs = sl.h.as<uint32_t>();
@@ -676,21 +828,80 @@ void prove_on(bg_pk* pk, ProveSlot& sl, const uint32_t* witness, const uint32_t*
       } else {
         msm_enqueue(1, pk->ph.as<uint32_t>(), sl.h.as<uint32_t>(), nh, 0, s0, sl.work[1]);
       }
-    } else {
-      BG_HIP(hipEventRecord(ev[5], s0));
-      if (pk->fixed) {
-        // schedule 1 batches H with A, B1, L: its table has its own width, so H runs alone
-        msm_enqueue_fixed(1, 3, g1p, g1s, g1n, pk->c_g1, s0, sl.work[0]);
-        const uint32_t* tb = pk->ph.as<uint32_t>();
-        const uint32_t* hs = sl.h.as<uint32_t>();
-        const uint64_t nn = nh;
-        msm_enqueue_fixed(1, 1, &tb, &hs, &nn, pk->c_h, s0, sl.work[1]);
-      } else {
-        msm_enqueue_batch(1, 4, g1p, g1s, g1n, 0, s0, sl.work[0]);
+    };
+    auto run_s0 = [&] {  // schedule 0's sequence on s0
+      run_evals();
+      run_qap();
+      run_h();
+      record_if_eager(ev[11], s0);
+    };
+    auto enqueue_device = [&] {
+      head();
+      if (sched == 0) {
+        run_b2();
+        run_g1_batch3();
+        run_s0();
+        return;
       }
-      BG_HIP(hipEventRecord(ev[6], s0));
+      run_evals();
+      if (sched == 4) {
+        run_qap();
+        BG_HIP(hipStreamWaitEvent(s3, ev[4], 0));
+        BG_HIP(hipStreamWaitEvent(s1, ev[4], 0));
+        run_b2();
+        run_g1_batch3();
+      } else {
+        run_b2();
+        if (sched == 2) {
+          BG_HIP(hipStreamWaitEvent(s1, sl.work[2].mid, 0));
+          run_g1_batch3();
+        }
+        run_qap();
+        if (sched == 3 || sched == 5) {
+          BG_HIP(hipStreamWaitEvent(s1, ev[4], 0));
+          run_g1_batch3();
+        }
+      }
+      if (sched != 1) {
+        run_h();
+      } else {
+        record_if_eager(ev[5], s0);
+        if (pk->fixed) {
+          // schedule 1 batches H with A, B1, L: its table has its own width, so H runs alone
+          msm_enqueue_fixed(1, 3, g1p, g1s, g1n, pk->c_g1, s0, sl.work[0]);
+          const uint32_t* tb = pk->ph.as<uint32_t>();
+          const uint32_t* hs = sl.h.as<uint32_t>();
+          const uint64_t nn = nh;
+          msm_enqueue_fixed(1, 1, &tb, &hs, &nn, pk->c_h, s0, sl.work[1]);
+        } else {
+          msm_enqueue_batch(1, 4, g1p, g1s, g1n, 0, s0, sl.work[0]);
+        }
+        record_if_eager(ev[6], s0);
+      }
+      record_if_eager(ev[11], s0);
+    };
+    const bool graph_ok = !dev_comb && sched == 0 && !ctx().timing && graph_wanted(lg);
+    bool graphed = false;
+    if (graph_ok) {
+      const uint64_t key = graph_key(pk, sl, sched);
+      const ProveSlot::Graph* g = sl.find_graph(key);
+      if (!g && sl.is_warm(key) && sl.failed_key != key) g = record_graphs(sl, key, run_b2, run_g1_batch3, run_s0);
+      if (g) {
+        head();
+        sl.work[0].arm_publish(s1);  // G1 batch
+        sl.work[1].arm_publish(s0);  // H
+        sl.work[2].arm_publish(s3);  // B2
+        BG_HIP(hipGraphLaunch(g->exec[0], s3));
+        BG_HIP(hipGraphLaunch(g->exec[1], s1));
+        BG_HIP(hipGraphLaunch(g->exec[2], s0));
+        g_graph_launches.fetch_add(1);
+        graphed = true;
+      }
     }
-    BG_HIP(hipEventRecord(ev[11], s0));
+    if (!graphed) {
+      enqueue_device();
+      if (graph_ok) sl.add_warm(graph_key(pk, sl, sched));  // the buffers now fit this key's proofs
+    }
 
     if (dev_comb) {
       // A, B1, L finished and combined once the G1 batch is reduced (its stream), B2 on its
@@ -793,10 +1004,11 @@ void prove_on(bg_pk* pk, ProveSlot& sl, const uint32_t* witness, const uint32_t*
       // device time of each phase on its own stream (the MSMs overlap), then the wall
       // time of the whole call
       // upload, evals, qap, G1 batch, (unused), B2, H alone (schedule 0) / G1 batch end
+      // (a replayed graph records no events: its device phases read -1)
       const int pairs[8][2] = {{0, 1}, {1, 3}, {3, 4}, {5, 6}, {5, 6}, {9, 10}, {4, 11}, {5, 6}};
       for (int i = 0; i < 8; i++) {
-        float e = 0;
-        BG_HIP(hipEventElapsedTime(&e, ev[pairs[i][0]], ev[pairs[i][1]]));
+        float e = -1;
+        if (!graphed || i == 0) BG_HIP(hipEventElapsedTime(&e, ev[pairs[i][0]], ev[pairs[i][1]]));
         t_ms[i] = e;
       }
       t_ms[8] = std::chrono::duration<double, std::milli>(t1 - t0).count();
@@ -965,4 +1177,13 @@ extern "C" int bg_proof_format(const uint32_t* p, char* buf, int cap) {
     BG_REQUIRE(n < cap, BG_ERR_ARG, "bg_proof_format: buffer of %d bytes too small", cap);
   });
   return rc == BG_OK ? n : rc;
+}
+
+extern "C" int bg_graph_info(uint64_t* info3) {
+  return guard([&] {
+    BG_REQUIRE(info3, BG_ERR_ARG, "bg_graph_info: null argument");
+    info3[0] = bg::g_graphs_recorded.load();
+    info3[1] = bg::g_graph_launches.load();
+    info3[2] = bg::g_graph_failures.load();
+  });
 }

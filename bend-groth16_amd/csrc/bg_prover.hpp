@@ -20,6 +20,35 @@ struct ProveSlot {
   HostBuf proofpin;  // pinned landing of a device-combined proof
   uint64_t cap_nz = 0, cap_d = 0;
   MsmWork work[3];  // G1 batch (A, B1, L), H, B2
+  // Recorded launch sequences of single proofs (prove_on): three linear HIP graphs (one per
+  // stream) per key, schedule, tuning table, BG16_ settings and buffer layout (`key`),
+  // least recently used dropped beyond MAX_GRAPHS.  A graph is recorded only when an
+  // earlier proof on this slot ran eagerly with the same key (same buffer addresses and
+  // sizes), so every buffer it needs already has its final size and nothing is allocated
+  // while recording.
+  struct Graph {
+    uint64_t key = 0;
+    hipGraphExec_t exec[3] = {};  // launched on s[3], s[1], s[0]
+    uint64_t used = 0;
+    // host side of the recorded MSMs (how msm_finish reads their outputs); put back into
+    // work[] on each launch, since other keys' proofs on this slot overwrite it
+    MsmPlan plan[3];
+    int group[3] = {};
+  };
+  static constexpr size_t MAX_GRAPHS = 8;
+  std::vector<Graph> graphs;
+  uint64_t graph_clock = 0;
+  std::vector<uint64_t> warm;  // keys of recent eager proofs on this slot (newest last)
+  uint64_t failed_key = 0;     // key whose recording failed (it runs eagerly)
+  bool is_warm(uint64_t key) const;
+  void add_warm(uint64_t key);
+  hipEvent_t join[3] = {};  // join the side streams back into s[0] at the end of a recording
+  // The graphs recorded for key with work[]'s host side restored to their recording, or
+  // null.
+  const Graph* find_graph(uint64_t key);
+  const Graph* add_graph(uint64_t key, const hipGraphExec_t exec[3]);  // keeps work[]'s host side too
+  void drop_graphs();
+  uint64_t buffer_signature() const;  // addresses and sizes of every work buffer
   ProveSlot();
   ~ProveSlot();
   ProveSlot(const ProveSlot&) = delete;

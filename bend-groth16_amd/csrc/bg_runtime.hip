@@ -64,6 +64,18 @@ struct Pending {
 };
 static std::vector<Pending> g_pending;
 
+bool stream_capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  BG_HIP(hipStreamIsCapturing(s, &st));
+  return st == hipStreamCaptureStatusActive;
+}
+
+bool record_if_eager(hipEvent_t e, hipStream_t s) {
+  if (stream_capturing(s)) return false;
+  BG_HIP(hipEventRecord(e, s));
+  return true;
+}
+
 void timing_record(const char* name, hipStream_t, hipEvent_t start, hipEvent_t stop) {
   std::lock_guard<std::mutex> lk(g_tmu);
   g_pending.push_back({name, start, stop});

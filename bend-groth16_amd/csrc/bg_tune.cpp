@@ -5,6 +5,7 @@
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <string>
@@ -23,6 +24,7 @@ struct Table {
 
 std::mutex g_mu;
 Table g_table;
+std::atomic<uint64_t> g_generation{1};
 
 std::string default_path() {
   const char* env = getenv("BG16_TUNE_FILE");
@@ -144,6 +146,8 @@ bool tune_ntt(int log2n, NttTune& out) {
   return true;
 }
 
+uint64_t tune_generation() { return g_generation.load(); }
+
 }  // namespace bg
 
 using namespace bg;
@@ -153,6 +157,7 @@ extern "C" int bg_tune_load(const char* path) {
     Table t = parse(path && path[0] ? std::string(path) : default_path());
     std::lock_guard<std::mutex> lk(g_mu);
     g_table = std::move(t);
+    g_generation.fetch_add(1);
   });
 }
 

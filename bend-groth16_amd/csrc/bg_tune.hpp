@@ -29,4 +29,8 @@ struct NttTune {
 bool tune_msm(int group, bool fixed, uint64_t total_n, MsmTune& out);
 bool tune_ntt(int log2n, NttTune& out);
 
+// Incremented by every bg_tune_load: launch sequences recorded for reuse (the prover's
+// graphs) are only valid for the table they were planned with.
+uint64_t tune_generation();
+
 }  // namespace bg

@@ -166,6 +166,15 @@ int bg_pk_device_bytes(const bg_pk* pk, uint64_t* bytes);
 int bg_prove(bg_pk* pk, const uint32_t* witness, const uint32_t* rs, uint32_t* proof_out,
              int flags, double* t_ms);
 
+/* Single proofs (bg_prove, and engine requests proved alone) of domains up to 2^15
+ * replay recorded HIP graphs of their device work after the upload, one linear graph per
+ * stream: the second proof of a key on a slot records them, later ones launch them
+ * (BG16_GRAPH=0 turns this off, =1 on for every size).  No counterpart in the reference
+ * (one process per proof); the proof is the same either way.  info3: graph sets recorded,
+ * proofs launched from graphs, recordings that failed (and ran eagerly), counted over
+ * the process. */
+int bg_graph_info(uint64_t* info3);
+
 /* ------------------------------------------------------------------ proving engine
  * Continuous batching of proofs on one device.  The reference proves one instance per
  * process (bend2/prove.bend:129-162, run by bend2/scripts/prove.sh); the engine is the
