@@ -674,6 +674,43 @@ def sizes_mode(a) -> dict:
             "n_gpus": 1, "sizes": out}
 
 
+def launch_mode(a) -> dict:
+    """Launch shapes the prover's recorded sequences choose between (bg_probe_launch): 96
+    short kernels eagerly or from HIP graphs, on one stream or forked over four, at two
+    kernel lengths; and single proofs by size with the recorded sequences off and on."""
+    bendg.init(0)
+    out = {}
+    for spin in (0, 2000):
+        out[f"spin_{spin}"] = bendg.probe_launch(96, spin, reps=40)
+        log(f"96 kernels, spin {spin}: {out[f'spin_{spin}']} us")
+    proofs = {}
+    hip = Hip()
+    for k in (10, 12, 14, 16, 18):
+        kdir = ensure_key(k, 42 + k, Path(a.keys))
+        pk = bendg.ProvingKey(kdir)
+        z, rs = witness_files(kdir)
+        dz = hip.upload(z)
+        row = {}
+        for g in ("0", "1", "0", "1"):  # alternating, twice
+            os.environ["BG16_GRAPH"] = g
+            for _ in range(3):
+                pk.prove_device(dz, rs)
+            walls = []
+            for _ in range(a.steps):
+                t0 = time.perf_counter()
+                pk.prove_device(dz, rs)
+                walls.append((time.perf_counter() - t0) * 1e3)
+            row.setdefault("eager" if g == "0" else "graphs", []).append(round(float(np.median(walls)), 3))
+        os.environ.pop("BG16_GRAPH", None)
+        proofs[str(k)] = row
+        log(f"K={k}: {row}")
+        hip.free(dz)
+        pk.close()
+    out["proof_ms_median"] = proofs
+    return {"metric": "launch probe (us per repetition) and single proofs eager vs recorded graphs", "value": None,
+            "unit": "us", "n_gpus": 1, **out}
+
+
 def tune_mode(a) -> dict:
     """Device-measured parameter table (replaces Msm.pick_c / pick_fd, bend2/src/msm.bend:37-44,
     and Ntt.fd, bend2/src/ntt.bend:382-385).  Each candidate runs with a one-entry table (plus
@@ -842,7 +879,8 @@ def main() -> int:
     ap.add_argument("--keys", default=os.environ.get("BG16_KEYS", "/tmp/bg16_keys"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mix", action="store_true", help="single mode: skip the 64-proof batched mix")
-    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve", "primitives", "roofline", "tune", "sizes"])
+    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve", "primitives", "roofline", "tune", "sizes",
+                                                               "launch"])
     ap.add_argument("--slots", type=lambda v: [int(x) for x in v.split(",")], default=[4, 8],
                     help="engine slots (batch: comma list to sweep)")
     ap.add_argument("--rates", type=lambda v: [float(x) for x in v.split(",")], default=[100.0, 200.0, 300.0],
@@ -862,7 +900,7 @@ def main() -> int:
         os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, max(1, hwq)))
     if a.mode != "single":
         line = {"batch": batch_mode, "serve": serve_mode, "primitives": primitives_mode,
-                "roofline": roofline_mode, "tune": tune_mode, "sizes": sizes_mode}[a.mode](a)
+                "roofline": roofline_mode, "tune": tune_mode, "sizes": sizes_mode, "launch": launch_mode}[a.mode](a)
         print(json.dumps(line), flush=True)
         return 0
 

@@ -76,6 +76,7 @@ def lib() -> ctypes.CDLL:
         "bg_field_op": [I, I, P, P, P, U64],
         "bg_curve_op": [I, I, P, P, P, U64],
         "bg_probe_rate": [I, U64, P],
+        "bg_probe_launch": [I, I, I, P],
         "bg_msm_plan_info": [U64, I, I, P],
         "bg_msm_batch": [I, ctypes.c_uint32, P, P, P, P, I, I],
         "bg_engine_create": [I, U64, ctypes.POINTER(ctypes.c_void_p)],
@@ -194,6 +195,17 @@ def probe_rate(kind: int, iters: int) -> tuple[float, float]:
     r = np.zeros(2, dtype=np.float64)
     _check(lib().bg_probe_rate(kind, iters, _p(r)))
     return float(r[0]), float(r[1])
+
+
+LAUNCH_VARIANTS = ("eager_linear", "graph_linear", "eager_fork", "graph_fork", "graph_fork_event_nodes",
+                   "graph_per_stream")
+
+
+def probe_launch(kernels: int, spin: int, reps: int = 40) -> dict:
+    """``bg_probe_launch``: microseconds per repetition of each launch shape."""
+    r = np.zeros(6, dtype=np.float64)
+    _check(lib().bg_probe_launch(kernels, spin, reps, _p(r)))
+    return {k: round(float(v), 1) for k, v in zip(LAUNCH_VARIANTS, r)}
 
 
 # ---------------------------------------------------------------- hot path

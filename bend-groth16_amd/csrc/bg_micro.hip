@@ -1,10 +1,11 @@
 // Known-answer kernels, element-wise field/curve kernels for the parity tests, and
 // throughput probes (C ABI: bg_kat_field_mul_xor, bg_kat_g1_add_chain, bg_field_op,
-// bg_curve_op, bg_probe_rate).
+// bg_curve_op, bg_probe_rate, bg_probe_launch).
 #include "bg_curve.hpp"
 #include "bg_runtime.hpp"
 #include <string.h>
 #include <chrono>
+#include <functional>
 
 namespace bg {
 
@@ -349,5 +350,121 @@ extern "C" int bg_probe_rate(int kind, uint64_t iters, double* result) {
     result[1] = ms;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+  });
+}
+
+// ---------------------------------------------------------------- launch probe
+// The cost of launching `kernels` short kernels (256 blocks of 64 threads, `spin` dependent
+// integer steps each) eagerly or from HIP graphs, the shapes the prover's recorded
+// sequences choose between: us6 = microseconds per repetition (launch to completion) of
+//   0 eager, one stream            1 one linear graph
+//   2 eager, fork to 4 streams      3 one graph that forks and joins the 4 streams
+//   4 as 3 with a host-visible event node (hipEventRecordExternal) on each side stream
+//   5 one linear graph per stream, launched after an eager fork
+namespace bg {
+static constexpr int PROBE_NB = 256;
+__global__ void k_probe_work(uint32_t* __restrict__ p, int i, int spin) {
+  const uint32_t k = blockIdx.x * 64 + threadIdx.x;
+  uint32_t v = p[k];
+  for (int j = 0; j < spin; j++) v = v * 1664525u + 1013904223u;
+  p[k] = v + (uint32_t)i;
+}
+}  // namespace bg
+
+extern "C" int bg_probe_launch(int kernels, int spin, int reps, double* us6) {
+  return guard([&] {
+    BG_REQUIRE(kernels >= 8 && kernels <= 4096 && spin >= 0 && spin <= 1000000 && reps >= 1 && reps <= 10000 && us6,
+               BG_ERR_ARG, "bg_probe_launch: bad argument");
+    Ctx& c = ctx();
+    uint32_t* d = c.scratch[1].as<uint32_t>(4ull * PROBE_NB * 64);
+    BG_HIP(hipMemsetAsync(d, 0, 16ull * PROBE_NB * 64, c.stream));
+    BG_HIP(hipStreamSynchronize(c.stream));
+    hipStream_t s[4] = {};
+    hipEvent_t fork = nullptr, join[3] = {}, ext[3] = {};
+    hipGraphExec_t ex[6] = {};
+    struct Cleanup {
+      hipStream_t* s;
+      hipEvent_t* fork;
+      hipEvent_t* join;
+      hipEvent_t* ext;
+      hipGraphExec_t* ex;
+      ~Cleanup() {
+        for (int i = 0; i < 6; i++)
+          if (ex[i]) (void)hipGraphExecDestroy(ex[i]);
+        if (*fork) (void)hipEventDestroy(*fork);
+        for (int i = 0; i < 3; i++) {
+          if (join[i]) (void)hipEventDestroy(join[i]);
+          if (ext[i]) (void)hipEventDestroy(ext[i]);
+        }
+        for (int i = 0; i < 4; i++)
+          if (s[i]) (void)hipStreamDestroy(s[i]);
+      }
+    } cleanup{s, &fork, join, ext, ex};
+    for (auto& x : s) BG_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+    BG_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    for (int i = 0; i < 3; i++) {
+      BG_HIP(hipEventCreateWithFlags(&join[i], hipEventDisableTiming));
+      BG_HIP(hipEventCreateWithFlags(&ext[i], hipEventDisableTiming));
+    }
+    const int per = kernels / 4;
+    auto linear = [&](hipStream_t st, uint32_t* p, int n) {
+      for (int i = 0; i < n; i++) k_probe_work<<<PROBE_NB, 64, 0, st>>>(p, i, spin);
+    };
+    auto forked = [&](bool ext_events) {
+      k_probe_work<<<PROBE_NB, 64, 0, s[0]>>>(d, 0, spin);
+      BG_HIP(hipEventRecord(fork, s[0]));
+      for (int b = 1; b < 4; b++) {
+        BG_HIP(hipStreamWaitEvent(s[b], fork, 0));
+        linear(s[b], d + (uint64_t)b * PROBE_NB * 64, per);
+        if (ext_events) BG_HIP(hipEventRecordWithFlags(ext[b - 1], s[b], hipEventRecordExternal));
+      }
+      linear(s[0], d, per);
+      for (int b = 1; b < 4; b++) {
+        BG_HIP(hipEventRecord(join[b - 1], s[b]));
+        BG_HIP(hipStreamWaitEvent(s[0], join[b - 1], 0));
+      }
+    };
+    auto record = [&](hipStream_t st, const std::function<void()>& fn) {
+      hipGraph_t g = nullptr;
+      hipGraphExec_t e = nullptr;
+      BG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      fn();
+      BG_HIP(hipStreamEndCapture(st, &g));
+      const hipError_t r = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      BG_HIP(r);
+      return e;
+    };
+    ex[0] = record(s[0], [&] { linear(s[0], d, kernels); });
+    ex[1] = record(s[0], [&] { forked(false); });
+    ex[2] = record(s[0], [&] { forked(true); });
+    for (int b = 0; b < 3; b++) ex[3 + b] = record(s[b + 1], [&] { linear(s[b + 1], d + (uint64_t)(b + 1) * PROBE_NB * 64, per); });
+    auto split = [&] {
+      k_probe_work<<<PROBE_NB, 64, 0, s[0]>>>(d, 0, spin);
+      BG_HIP(hipEventRecord(fork, s[0]));
+      for (int b = 1; b < 4; b++) {
+        BG_HIP(hipStreamWaitEvent(s[b], fork, 0));
+        BG_HIP(hipGraphLaunch(ex[2 + b], s[b]));
+      }
+      linear(s[0], d, per);
+      for (int b = 1; b < 4; b++) {
+        BG_HIP(hipEventRecord(join[b - 1], s[b]));
+        BG_HIP(hipStreamWaitEvent(s[0], join[b - 1], 0));
+      }
+    };
+    const std::function<void()> variants[6] = {
+        [&] { linear(s[0], d, kernels); },       [&] { BG_HIP(hipGraphLaunch(ex[0], s[0])); },
+        [&] { forked(false); },                   [&] { BG_HIP(hipGraphLaunch(ex[1], s[0])); },
+        [&] { BG_HIP(hipGraphLaunch(ex[2], s[0])); }, split};
+    for (int v = 0; v < 6; v++) {
+      variants[v]();  // warm-up
+      BG_HIP(hipStreamSynchronize(s[0]));
+      const auto t0 = std::chrono::steady_clock::now();
+      for (int r = 0; r < reps; r++) {
+        variants[v]();
+        BG_HIP(hipStreamSynchronize(s[0]));
+      }
+      us6[v] = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count() / reps;
+    }
   });
 }

@@ -622,9 +622,9 @@ void check_host_inputs(const uint32_t* witness, const uint32_t* rs, int flags, u
 
 // Single proofs replay recorded HIP graphs of their device work (BG16_GRAPH: 0 never,
 // 1 always; unset: domains up to 2^GRAPH_DEFAULT_MAX_LOG2).  Measured per proof, eager vs
-// replayed: K=10 1.005 / 0.989 ms, K=12 1.117 / 1.094, K=14 1.340 / 1.296, K=16 1.856 /
-// 1.925, K=18 4.066 / 4.188 -- small proofs gain the host's launch work, large ones lose
-// a little to the graphs' own dispatch.
+// replayed: K=10 0.99 / 0.98 ms, K=12 1.14 / 1.11, K=14 1.35 / 1.32, K=16 1.85 / 1.93,
+// K=18 4.02 / 4.17 -- small proofs gain the host's launch work, large ones lose a little
+// to the graphs' own dispatch.
 static constexpr uint32_t GRAPH_DEFAULT_MAX_LOG2 = 15;
 static bool graph_wanted(uint32_t lg) {
   const char* e = getenv("BG16_GRAPH");

@@ -255,6 +255,12 @@ int bg_curve_op(int group, int op, const uint32_t* p, const uint32_t* q, uint32_
 /* Throughput probes: kind 0 = v_mad_u64_u32 issue rate (ops/s), 1 = Fq Montgomery
  * products/s, 2 = G1 XYZZ mixed adds/s.  result[0] = rate, result[1] = checksum. */
 int bg_probe_rate(int kind, uint64_t iters, double* result);
+/* Launch probe (no counterpart in the reference): microseconds per repetition of `kernels`
+ * short kernels (`spin` dependent steps each) launched 0 eagerly on one stream, 1 from one
+ * linear graph, 2 eagerly forked over four streams, 3 from one graph that forks and joins
+ * them, 4 as 3 with host-visible event nodes, 5 from one linear graph per stream after an
+ * eager fork: the choice behind the prover's recorded launch sequences. */
+int bg_probe_launch(int kernels, int spin, int reps, double* us6);
 
 #ifdef __cplusplus
 }
