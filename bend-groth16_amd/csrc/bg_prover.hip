@@ -397,6 +397,7 @@ ProveSlot::ProveSlot() {
   for (auto& x : s) BG_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
   for (auto& e : ev) BG_HIP(hipEventCreate(&e));
   for (auto& e : join) BG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  graphs.reserve(MAX_GRAPHS);  // entries never move while a pointer to one is in use
 }
 
 ProveSlot::~ProveSlot() {

@@ -96,6 +96,31 @@ def test_graph_equals_eager_with_phase_times(bg, gkeys, monkeypatch):
     pk.close()
 
 
+def test_graph_device_witness(bg, gkeys, monkeypatch):
+    """a packed witness already in HBM (BG_MEM_DEVICE): replays read each new device witness"""
+    from helpers import DeviceBuffers
+    monkeypatch.setenv("BG16_GRAPH", "1")
+    d = gkeys[12]
+    inst = load(d)
+    n = int(inst["header"][0])
+    pk = bendg.ProvingKey(d)
+    rp = _packed(np.fromfile(d / "rs.bin", dtype=np.uint32))
+    before = bendg.graph_info()
+    rnd = random.Random(23)
+    dev = DeviceBuffers()
+    try:
+        for i in range(4):  # a new device buffer each time
+            z = chain_witness(n, rnd.randrange(1, R))
+            ptr = dev.upload(z.view(np.uint32).reshape(-1))
+            proof, _ = pk.prove_device(ptr, rp)
+            assert np.array_equal(proof.view(np.uint64), _expect(inst, z)), f"device witness proof {i}"
+    finally:
+        dev.free_all()
+    after = bendg.graph_info()
+    assert after["recorded"] - before["recorded"] == 1 and after["launches"] - before["launches"] == 3
+    pk.close()
+
+
 def test_engine_slot_alternating_keys(bg, gkeys, monkeypatch):
     """keys of three sizes take turns on one engine slot: each keeps its own graph and
     every proof equals the eager proof made alone"""
