@@ -186,6 +186,92 @@ struct Curve {
   }
 };
 
+// ---------------------------------------------------------------- quad-cooperative sums
+// For the latency-bound reductions (few waves in flight, each waiting on a chain of
+// dependent additions) a sum gets the four lanes of a quad (lanes 4i .. 4i+3 of a wave):
+// every lane holds both operands, computes one of the four independent products of a step
+// on operands selected by its lane index -- one product's instruction stream yields four
+// products -- and the four products are shared by DPP quad broadcasts.  The add-2008-s
+// sum then takes four product steps instead of 14 products, the dbl-2008-s-1 doubling
+// four instead of eight; every lane ends with the whole result.  The products are the same
+// field operations as Curve's (the same group element; representatives may differ within
+// [0, 2p)).  Callers keep control flow uniform within each quad.
+// (by masks: a select between whole operands lets the compiler select their addresses and
+// keep them in private memory)
+__device__ __forceinline__ Fp qsel(uint32_t r, const Fp& a0, const Fp& a1, const Fp& a2, const Fp& a3) {
+  const uint32_t m0 = 0u - (uint32_t)(r == 0), m1 = 0u - (uint32_t)(r == 1), m2 = 0u - (uint32_t)(r == 2),
+                 m3 = 0u - (uint32_t)(r == 3);
+  Fp o;
+#pragma unroll
+  for (int i = 0; i < 8; i++) o.v[i] = (a0.v[i] & m0) | (a1.v[i] & m1) | (a2.v[i] & m2) | (a3.v[i] & m3);
+  return o;
+}
+__device__ __forceinline__ Fp2 qsel(uint32_t r, const Fp2& a0, const Fp2& a1, const Fp2& a2, const Fp2& a3) {
+  return {qsel(r, a0.c0, a1.c0, a2.c0, a3.c0), qsel(r, a0.c1, a1.c1, a2.c1, a3.c1)};
+}
+// lane S of each quad to all four (DPP quad_perm [S, S, S, S])
+template <int S>
+__device__ __forceinline__ Fp qbc(const Fp& x) {
+  Fp o;
+#pragma unroll
+  for (int i = 0; i < 8; i++) o.v[i] = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x.v[i], S * 0x55, 0xF, 0xF, false);
+  return o;
+}
+template <int S>
+__device__ __forceinline__ Fp2 qbc(const Fp2& x) {
+  return {qbc<S>(x.c0), qbc<S>(x.c1)};
+}
+
+template <class F>
+struct Quad {
+  using C = Curve<F>;
+  using E = typename F::E;
+  using P = typename C::P;
+  __device__ __forceinline__ static uint32_t role() { return threadIdx.x & 3u; }
+
+  // Curve::add_lite on a quad (p, q the same in its four lanes)
+  __device__ __forceinline__ static P add_lite(const P& p, const P& q, uint32_t& bad) {
+    if (C::is_inf(p)) return q;
+    if (C::is_inf(q)) return p;
+    const uint32_t r = role();
+    // u1 = X1 ZZ2, u2 = X2 ZZ1, s1 = Y1 ZZZ2, s2 = Y2 ZZZ1
+    E m = F::mul(qsel(r, p.X, q.X, p.Y, q.Y), qsel(r, q.ZZ, p.ZZ, q.ZZZ, p.ZZZ));
+    const E u1 = qbc<0>(m), u2 = qbc<1>(m), s1 = qbc<2>(m), s2 = qbc<3>(m);
+    const E pp_ = F::sub(u2, u1), rr = F::sub(s2, s1);
+    bad |= F::is_zero(pp_) ? 1u : 0u;
+    // pp = P^2, r^2, ZZ1 ZZ2, ZZZ1 ZZZ2
+    m = F::mul(qsel(r, pp_, rr, p.ZZ, p.ZZZ), qsel(r, pp_, rr, q.ZZ, q.ZZZ));
+    const E pp = qbc<0>(m), r2 = qbc<1>(m), zz12 = qbc<2>(m), zzz12 = qbc<3>(m);
+    // ppp = P pp, q = u1 pp, ZZ3 = ZZ1 ZZ2 pp
+    m = F::mul(qsel(r, pp_, u1, zz12, zz12), pp);
+    const E ppp = qbc<0>(m), q_ = qbc<1>(m), zz3 = qbc<2>(m);
+    const E x3 = F::sub(F::sub(r2, ppp), F::dbl(q_));
+    // Y3 = r (q - X3) - s1 ppp, ZZZ3 = ZZZ1 ZZZ2 ppp - 0
+    const E z = F::zero(), t = F::sub(q_, x3);
+    m = F::mul_sub(qsel(r, rr, zzz12, zzz12, zzz12), qsel(r, t, ppp, ppp, ppp), qsel(r, s1, z, z, z), ppp);
+    return {x3, qbc<0>(m), zz3, qbc<1>(m)};
+  }
+
+  // Curve::dbl on a quad
+  __device__ __forceinline__ static P dbl(const P& p) {
+    const uint32_t r = role();
+    const E u = F::dbl(p.Y);
+    // v = u^2, x2 = X^2
+    E m = F::mul(qsel(r, u, p.X, u, u), qsel(r, u, p.X, u, u));
+    const E v = qbc<0>(m), x2 = qbc<1>(m);
+    // w = u v, s = X v, ZZ3 = v ZZ
+    m = F::mul(qsel(r, u, p.X, p.ZZ, p.ZZ), v);
+    const E w = qbc<0>(m), s = qbc<1>(m), zz3 = qbc<2>(m);
+    const E mm = F::add(F::dbl(x2), x2);
+    // mm^2, ZZZ3 = w ZZZ
+    m = F::mul(qsel(r, mm, w, w, w), qsel(r, mm, p.ZZZ, p.ZZZ, p.ZZZ));
+    const E m2 = qbc<0>(m), zzz3 = qbc<1>(m);
+    const E x3 = F::sub(m2, F::dbl(s));
+    const E y3 = F::mul_sub(mm, F::sub(s, x3), w, p.Y);
+    return {x3, y3, zz3, zzz3};
+  }
+};
+
 using G1 = Curve<Fq>;
 using G2 = Curve<Fq2>;
 

@@ -32,6 +32,7 @@ static int tune_env(const char* name) {
 
 uint64_t accumulate_threads_g1();
 uint64_t accumulate_threads_g2();
+uint32_t reduce2_quad_kmax_g1();
 uint64_t accumulate_threads_per_round(int group) {
   return group == 1 ? accumulate_threads_g1() : accumulate_threads_g2();
 }
@@ -65,6 +66,15 @@ static void finish_plan(MsmPlan& p, int group) {
   // work cancels it)
   uint32_t kmax = group == 1 ? 256 : 128;
   if (group == 2 && p.W == 1 && p.G <= 1024) kmax = 32;
+  // G1's level 2 with a quad per group (k_reduce2q, blocks as wide as that kernel takes)
+  // where the level is latency-bound: at most 2^15 groups (a windowed 2^19 MSM's 69,632
+  // groups fill the device and measured 0.235 -> 0.254 ms with quads) and batches of at
+  // least 2^16 points (smaller proofs replay recorded launch sequences, with which the
+  // 2^14 proof measured 1.30 -> 1.35 ms).  Measured per proof: level-2 time 35-42 % lower
+  // at every size; eager proofs K=10 1.02-1.06 -> 0.98-1.00 ms, K=18 4.01-4.05 -> 3.98-4.01.
+  p.quad2 = group == 1 && reduce2_quad_kmax_g1() != 0 && p.n >= (1u << 16) &&
+            (uint64_t)p.M * p.W * p.G <= (1u << 15);
+  if (p.quad2) kmax = std::min(kmax, reduce2_quad_kmax_g1());
   // override: rounded down to a power of two (the level-2 scan and trees need one, and
   // K must divide the group count)
   uint32_t kk = (uint32_t)std::max(0, tune_env("BG16_MSM_K"));

@@ -35,6 +35,7 @@ struct MsmPlan {
   uint64_t n = 0;     // total points
   uint64_t entries = 0;  // Wd * total points
   uint32_t acc_threads = 0;
+  bool quad2 = false;  // level 2 with a quad per group (k_reduce2q)
 };
 
 MsmPlan msm_plan(uint64_t n, int group, int window_bits);

@@ -440,6 +440,97 @@ __global__ __launch_bounds__(256) void k_reduce2(const uint32_t* __restrict__ lv
   }
 }
 
+// k_reduce2 (CAREFUL = false) with a quad per level-1 group (blockDim.x = 4 K): the same
+// scan, trees and outputs, each addition and doubling spread over the quad's four lanes
+// (Quad), so the block's chain of dependent additions takes about half the time.
+// G1 only: blocks of up to 512 threads (K = 128 groups; at 1024 a quad's G1 sum would
+// spill past 128 VGPRs); a G2 sum would need more than a lane's 256 VGPRs (the B2
+// reductions are off a proof's critical path and keep the one-lane kernel).
+template <class F>
+__global__ __launch_bounds__(512) void k_reduce2q(const uint32_t* __restrict__ lvlS, const uint32_t* __restrict__ lvlT,
+                                                   int log2m, uint32_t* __restrict__ outPQ, uint32_t* __restrict__ rl2) {
+  using C = Curve<F>;
+  using Q = Quad<F>;
+  using P = typename C::P;
+  extern __shared__ __align__(16) unsigned char red2_smem[];
+  const uint32_t K = blockDim.x >> 2;
+  P* shA = reinterpret_cast<P*>(red2_smem);
+  P* shU = shA + K;
+  __shared__ uint32_t blk_bad;
+  const uint32_t blk = blockIdx.x;
+  const uint32_t k = threadIdx.x >> 2;  // this quad's group
+  const bool lead = (threadIdx.x & 3u) == 0;
+  uint32_t bad = 0;
+  if (threadIdx.x == 0) blk_bad = 0;
+  const uint64_t g = (uint64_t)blk * K + k;
+  const P S = C::load(lvlS + (uint64_t)C::xyzz_words * g);
+  const P T = C::load(lvlT + (uint64_t)C::xyzz_words * g);
+  P a = S;
+  const P u = Q::add_lite(T, S, bad);
+  if (lead) {
+    shA[k] = a;
+    shU[k] = u;
+  }
+  __syncthreads();
+  for (uint32_t d = 1; d < K; d <<= 1) {
+    P x = (k + d < K) ? shA[k + d] : C::infinity();
+    __syncthreads();
+    a = Q::add_lite(a, x, bad);
+    if (lead) shA[k] = a;
+    __syncthreads();
+  }
+  const P Z = shA[0];
+  __syncthreads();
+  if (threadIdx.x == 0) shA[0] = C::infinity();
+  __syncthreads();
+  if (K >= 2) {
+    const uint32_t half = K / 2, t = k % half;
+    P* arr = k < half ? shA : shU;
+    for (uint32_t s = half; s > 0; s >>= 1) {
+      // a quad reads arr[t] and arr[t + s] (t + s is written by no quad this round)
+      // before its lead lane writes arr[t]
+      if (t < s) {
+        const P v = Q::add_lite(arr[t], arr[t + s], bad);
+        if (lead) arr[t] = v;
+      }
+      __syncthreads();
+    }
+  }
+  if (k == 0) {
+    P V = shA[0];
+    for (int i = 0; i < log2m; i++) V = Q::dbl(V);
+    const P out = Q::add_lite(shU[0], V, bad);
+    if (lead) {
+      C::store(outPQ + (uint64_t)C::xyzz_words * (2 * blk), C::canon(out));
+      C::store(outPQ + (uint64_t)C::xyzz_words * (2 * blk + 1), C::canon(Z));
+    }
+  }
+  if (bad) atomicOr(&blk_bad, 1u);
+  __syncthreads();
+  if (threadIdx.x == 0 && blk_bad) rl2[1 + atomicAdd(&rl2[0], 1u)] = blk;
+}
+
+// Widest level-2 block (groups) the quad kernel can take: a quad per group, as many
+// threads as its registers allow in one block (0 when BG16_QUAD=0).
+template <class F>
+uint32_t reduce2_quad_kmax() {
+  if constexpr (sizeof(typename Curve<F>::P) > 128) return 0u;
+  static const uint32_t cached = [] {
+    const char* e = getenv("BG16_QUAD");
+    if (e && atoi(e) == 0) return 0u;
+    hipFuncAttributes attr;
+    if (hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(&k_reduce2q<Fq>)) != hipSuccess ||
+        attr.maxThreadsPerBlock < 8) {
+      (void)hipGetLastError();
+      return 0u;
+    }
+    uint32_t k = 1;
+    while (8 * k <= (uint32_t)attr.maxThreadsPerBlock) k *= 2;
+    return k;
+  }();
+  return cached;
+}
+
 template <class F>
 uint64_t accumulate_round_threads() {
   static uint64_t cached = [] {
@@ -602,7 +693,14 @@ void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* 
         }
       }
       ScopedKTime kt(group == 1 ? "msm_g1_reduce2" : "msm_g2_reduce2", s);
-      k_reduce2<F, false><<<nblocks, p.K, smem, s>>>(lvlS, lvlT, log2m, outPQ, rl2);
+      bool quad = false;
+      if constexpr (sizeof(typename C::P) <= 128) {
+        if (p.quad2 && p.K <= reduce2_quad_kmax<F>() && p.K >= 2) {
+          k_reduce2q<F><<<nblocks, 4 * p.K, smem, s>>>(lvlS, lvlT, log2m, outPQ, rl2);
+          quad = true;
+        }
+      }
+      if (!quad) k_reduce2<F, false><<<nblocks, p.K, smem, s>>>(lvlS, lvlT, log2m, outPQ, rl2);
       k_reduce2<F, true><<<8, p.K, smem, s>>>(lvlS, lvlT, log2m, outPQ, rl2);
       BG_HIP(hipGetLastError());
     }
