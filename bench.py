@@ -290,36 +290,48 @@ def batched_mix(a, hip, slots_list) -> dict:
     wz = [witness_files(d) for d in dirs]
     dzs = [hip.upload(z) for z, _ in wz]
     log(f"64 keys ready in {time.perf_counter() - t:.1f} s, K = {ks}")
-    # one proof at a time (the single-proof path), after one warm proof per key
+    # one proof at a time (the single-proof path), after two warm proofs per key (the
+    # second records the recorded launch sequences of the sizes that replay them, so the
+    # timed proofs replay them as a serving loop would)
     ref = [pk.prove_device(dz, rs)[0] for pk, dz, (_, rs) in zip(pks, dzs, wz)]
+    for pk, dz, (_, rs) in zip(pks, dzs, wz):
+        pk.prove_device(dz, rs)
     hip.sync()
-    per = []
-    t0 = time.perf_counter()
-    for pk, dz, (_, rs), r in zip(pks, dzs, wz, ref):
-        s0 = time.perf_counter()
-        p, _ = pk.prove_device(dz, rs)
-        per.append(time.perf_counter() - s0)
-        if not np.array_equal(p, r):
-            raise SystemExit("single proof not reproducible")
-    single_s = time.perf_counter() - t0
+    passes = []
+    for _ in range(3):  # the median of three passes over the 64 proofs
+        per = []
+        t0 = time.perf_counter()
+        for pk, dz, (_, rs), r in zip(pks, dzs, wz, ref):
+            s0 = time.perf_counter()
+            p, _ = pk.prove_device(dz, rs)
+            per.append(time.perf_counter() - s0)
+            if not np.array_equal(p, r):
+                raise SystemExit("single proof not reproducible")
+        passes.append((time.perf_counter() - t0, per))
+    single_s, per = sorted(passes, key=lambda x: x[0])[1]
     res = {}
     for slots in slots_list:
         eng = bendg.Engine(slots=slots)
-        for t in [eng.submit(pk, dz, rs) for pk, dz, (_, rs) in zip(pks, dzs, wz)]:
-            eng.wait(t)  # warm: every slot's work arrays at their largest
-        t0 = time.perf_counter()
-        tickets = [eng.submit(pk, dz, rs) for pk, dz, (_, rs) in zip(pks, dzs, wz)]
-        lat = []
-        for t, r in zip(tickets, ref):
-            p, tm = eng.wait(t)
-            lat.append(tm["latency"])
-            if not np.array_equal(p, r):
-                raise SystemExit("batched proof differs from the single proof")
-        dt = time.perf_counter() - t0
+        for _ in range(2):  # warm: every slot's work arrays at their largest
+            for t in [eng.submit(pk, dz, rs) for pk, dz, (_, rs) in zip(pks, dzs, wz)]:
+                eng.wait(t)
+        runs = []
+        for _ in range(3):  # the median of three passes of all 64 (one pass is ~0.1 s)
+            t0 = time.perf_counter()
+            tickets = [eng.submit(pk, dz, rs) for pk, dz, (_, rs) in zip(pks, dzs, wz)]
+            lat = []
+            for t, r in zip(tickets, ref):
+                p, tm = eng.wait(t)
+                lat.append(tm["latency"])
+                if not np.array_equal(p, r):
+                    raise SystemExit("batched proof differs from the single proof")
+            runs.append((time.perf_counter() - t0, lat))
         eng.close()
+        dt, lat = sorted(runs, key=lambda x: x[0])[1]
         res[slots] = {"proofs_per_s": round(64 / dt, 2), "wall_ms": round(dt * 1e3, 1),
                       "p50_ms": round(float(np.percentile(lat, 50)), 2),
-                      "p99_ms": round(float(np.percentile(lat, 99)), 2)}
+                      "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                      "passes_wall_ms": [round(x[0] * 1e3, 1) for x in runs]}
         log(f"engine slots={slots}: {res[slots]}")
     for pk in pks:
         pk.close()
@@ -998,7 +1010,7 @@ def main() -> int:
     mix = None
     if not a.no_mix:
         log("batched mix: 64 proofs, K uniform in 10..18, own keys")
-        mix = batched_mix(a, hip, [4])
+        mix = batched_mix(a, hip, [2, 3, 4])
     line = {
         "metric": METRIC, "value": round(ms, 3), "unit": "ms/proof", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(ms, 3), "higher_is_better": False, "scaling": "weak",
