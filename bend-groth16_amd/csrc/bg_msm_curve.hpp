@@ -345,6 +345,61 @@ __global__ __launch_bounds__(256) void k_reduce1(const uint32_t* __restrict__ bu
   }
 }
 
+// k_reduce1 (CAREFUL = false) with a quad per group: the same roles (one per wave), 16
+// groups per block, each addition and doubling spread over the quad's four lanes (Quad).
+template <class F>
+__global__ __launch_bounds__(256) void k_reduce1q(const uint32_t* __restrict__ buckets,
+                                                  const uint32_t* __restrict__ offsets, uint32_t B, uint32_t logm,
+                                                  uint32_t ngroups, uint32_t* __restrict__ lvlS,
+                                                  uint32_t* __restrict__ lvlT, uint32_t* mq, uint32_t* __restrict__ rl) {
+  using C = Curve<F>;
+  using Q = Quad<F>;
+  using P = typename C::P;
+  __shared__ uint32_t shbad[16];
+  const uint64_t xw = C::xyzz_words;
+  const uint32_t m = 1u << logm, G = B >> logm;
+  const uint32_t q = threadIdx.x >> 6, gl = (threadIdx.x & 63) >> 2;  // role = wave, group = quad
+  const bool lead = (threadIdx.x & 3u) == 0;
+  const uint64_t mstride = (uint64_t)gridDim.x * 16;  // scratch slots per role
+  for (uint32_t g0 = blockIdx.x * 16; g0 < ngroups; g0 += gridDim.x * 16) {
+    const uint32_t g = g0 + gl;
+    const bool valid = g < ngroups;
+    const uint32_t w = valid ? g / G : 0, j = valid ? g % G : 0;
+    const uint64_t base = (uint64_t)w * B + (uint64_t)j * m;
+    const uint64_t slot = (uint64_t)blockIdx.x * 16 + gl;
+    uint32_t bad = 0;
+    if (q == 0 && lead) shbad[gl] = 0;
+    if (valid && q <= logm) {
+      P acc = C::infinity();
+      for (uint32_t l = 0; l < m; l++) {
+        const bool take = q < logm ? ((l >> q) & 1u) != 0 : l < m / 2;
+        if (!take) continue;
+        const uint64_t key = base + l;
+        if (offsets[key + 1] > offsets[key]) acc = Q::add_lite(acc, C::load(buckets + xw * key), bad);
+      }
+      if (lead) C::store(mq + xw * (q * mstride + slot), acc);
+    }
+    __syncthreads();
+    if (valid && q == 0) {
+      P T = C::load(mq + xw * ((logm - 1) * mstride + slot));
+      for (int qq = (int)logm - 2; qq >= 0; qq--) {
+        T = Q::dbl(T);
+        T = Q::add_lite(T, C::load(mq + xw * (qq * mstride + slot)), bad);
+      }
+      if (lead) C::store(lvlT + xw * g, T);
+    } else if (valid && q == 1) {
+      const P hi = C::load(mq + xw * ((logm - 1) * mstride + slot));
+      const P lo = C::load(mq + xw * (logm * mstride + slot));
+      const P sum = Q::add_lite(hi, lo, bad);
+      if (lead) C::store(lvlS + xw * g, sum);
+    }
+    if (bad) shbad[gl] = 1;
+    __syncthreads();
+    if (valid && q == 0 && lead && shbad[gl]) rl[1 + atomicAdd(&rl[0], 1u)] = g;
+    __syncthreads();
+  }
+}
+
 // Level 1 with one lane per group (running sums from the top bucket, depth 2m): used
 // for G2, whose additions hold a whole register file per lane, so four role waves per
 // group would not fit the device in one round.
@@ -684,10 +739,20 @@ void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* 
           k_reduce1_serial<F, true><<<16, 64, 0, s>>>(buckets, offsets, p.B, p.m, (uint32_t)ngroups, lvlS, lvlT,
                                                       rl1);
         } else {
-          // one wave per role, 64 groups per block
-          const unsigned r1b = (unsigned)((ngroups + 63) / 64);
-          k_reduce1<F, false><<<r1b, 256, 0, s>>>(buckets, offsets, p.B, (uint32_t)log2m, (uint32_t)ngroups, lvlS,
-                                                  lvlT, mq, rl1);
+          // one wave per role, 64 groups per block (16 with a quad per group)
+          bool quad = false;
+          if constexpr (sizeof(typename C::P) <= 128) {
+            if (p.quad2) {
+              k_reduce1q<F><<<(unsigned)((ngroups + 15) / 16), 256, 0, s>>>(buckets, offsets, p.B, (uint32_t)log2m,
+                                                                            (uint32_t)ngroups, lvlS, lvlT, mq, rl1);
+              quad = true;
+            }
+          }
+          if (!quad) {
+            const unsigned r1b = (unsigned)((ngroups + 63) / 64);
+            k_reduce1<F, false><<<r1b, 256, 0, s>>>(buckets, offsets, p.B, (uint32_t)log2m, (uint32_t)ngroups, lvlS,
+                                                    lvlT, mq, rl1);
+          }
           k_reduce1<F, true><<<4, 256, 0, s>>>(buckets, offsets, p.B, (uint32_t)log2m, (uint32_t)ngroups, lvlS,
                                                lvlT, mq, rl1);
         }
