@@ -33,6 +33,7 @@ static int tune_env(const char* name) {
 uint64_t accumulate_threads_g1();
 uint64_t accumulate_threads_g2();
 uint32_t reduce2_quad_kmax_g1();
+uint32_t reduce2_quad_kmax_g2();
 uint64_t accumulate_threads_per_round(int group) {
   return group == 1 ? accumulate_threads_g1() : accumulate_threads_g2();
 }
@@ -72,9 +73,9 @@ static void finish_plan(MsmPlan& p, int group) {
   // least 2^16 points (smaller proofs replay recorded launch sequences, with which the
   // 2^14 proof measured 1.30 -> 1.35 ms).  Measured per proof: level-2 time 35-42 % lower
   // at every size; eager proofs K=10 1.02-1.06 -> 0.98-1.00 ms, K=18 4.01-4.05 -> 3.98-4.01.
-  p.quad2 = group == 1 && reduce2_quad_kmax_g1() != 0 && p.n >= (1u << 16) &&
-            (uint64_t)p.M * p.W * p.G <= (1u << 15);
-  if (p.quad2) kmax = std::min(kmax, reduce2_quad_kmax_g1());
+  const uint32_t kq = group == 1 ? reduce2_quad_kmax_g1() : reduce2_quad_kmax_g2();
+  p.quad2 = kq != 0 && p.n >= (1u << 16) && (uint64_t)p.M * p.W * p.G <= (1u << 15);
+  if (p.quad2) kmax = std::min(kmax, kq);
   // override: rounded down to a power of two (the level-2 scan and trees need one, and
   // K must divide the group count)
   uint32_t kk = (uint32_t)std::max(0, tune_env("BG16_MSM_K"));

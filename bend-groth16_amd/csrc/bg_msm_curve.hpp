@@ -431,6 +431,37 @@ __global__ __launch_bounds__(256) void k_reduce1_serial(const uint32_t* __restri
   }
 }
 
+// k_reduce1_serial (CAREFUL = false) with a quad per group (the running sums' additions
+// spread over the quad's four lanes).
+template <class F>
+__global__ __launch_bounds__(256) void k_reduce1_serialq(const uint32_t* __restrict__ buckets,
+                                                         const uint32_t* __restrict__ offsets, uint32_t B, uint32_t m,
+                                                         uint32_t ngroups, uint32_t* __restrict__ lvlS,
+                                                         uint32_t* __restrict__ lvlT, uint32_t* __restrict__ rl) {
+  using C = Curve<F>;
+  using Q = Quad<F>;
+  const bool lead = (threadIdx.x & 3u) == 0;
+  const uint32_t stride = (gridDim.x * blockDim.x) >> 2;
+  for (uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) >> 2; g < ngroups; g += stride) {
+    const uint32_t G = B / m;
+    const uint32_t w = g / G, j = g % G;
+    const uint64_t base = (uint64_t)w * B + (uint64_t)j * m;
+    typename C::P R = C::infinity(), T = C::infinity();
+    uint32_t bad = 0;
+    for (int l = (int)m - 1; l >= 0; l--) {
+      const uint64_t key = base + l;
+      if (offsets[key + 1] > offsets[key])
+        R = Q::add_lite(R, C::load(buckets + (uint64_t)C::xyzz_words * key), bad);
+      if (l > 0) T = Q::add_lite(T, R, bad);
+    }
+    if (lead) {
+      C::store(lvlS + (uint64_t)C::xyzz_words * g, R);
+      C::store(lvlT + (uint64_t)C::xyzz_words * g, T);
+      if (bad) rl[1 + atomicAdd(&rl[0], 1u)] = g;
+    }
+  }
+}
+
 // Level 2: a block of K = blockDim.x threads (power of two) takes K consecutive level-1
 // groups of one window:
 //   U = sum_k (T_k + S_k),  V = sum_k k*S_k (suffix scan + tree),  Z = sum_k S_k
@@ -498,11 +529,15 @@ __global__ __launch_bounds__(256) void k_reduce2(const uint32_t* __restrict__ lv
 // k_reduce2 (CAREFUL = false) with a quad per level-1 group (blockDim.x = 4 K): the same
 // scan, trees and outputs, each addition and doubling spread over the quad's four lanes
 // (Quad), so the block's chain of dependent additions takes about half the time.
-// G1 only: blocks of up to 512 threads (K = 128 groups; at 1024 a quad's G1 sum would
-// spill past 128 VGPRs); a G2 sum would need more than a lane's 256 VGPRs (the B2
-// reductions are off a proof's critical path and keep the one-lane kernel).
+// Blocks of up to 512 threads for G1 (K = 128 groups; at 1024 a quad's G1 sum would
+// spill past 128 VGPRs) and 256 for G2 (K = 64: a quad's G2 doubling holds ~210 VGPRs).
 template <class F>
-__global__ __launch_bounds__(512) void k_reduce2q(const uint32_t* __restrict__ lvlS, const uint32_t* __restrict__ lvlT,
+struct QuadThreads {
+  static constexpr int value = sizeof(typename Curve<F>::P) > 128 ? 256 : 512;
+};
+
+template <class F>
+__global__ __launch_bounds__(QuadThreads<F>::value) void k_reduce2q(const uint32_t* __restrict__ lvlS, const uint32_t* __restrict__ lvlT,
                                                    int log2m, uint32_t* __restrict__ outPQ, uint32_t* __restrict__ rl2) {
   using C = Curve<F>;
   using Q = Quad<F>;
@@ -569,12 +604,11 @@ __global__ __launch_bounds__(512) void k_reduce2q(const uint32_t* __restrict__ l
 // threads as its registers allow in one block (0 when BG16_QUAD=0).
 template <class F>
 uint32_t reduce2_quad_kmax() {
-  if constexpr (sizeof(typename Curve<F>::P) > 128) return 0u;
   static const uint32_t cached = [] {
     const char* e = getenv("BG16_QUAD");
     if (e && atoi(e) == 0) return 0u;
     hipFuncAttributes attr;
-    if (hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(&k_reduce2q<Fq>)) != hipSuccess ||
+    if (hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(&k_reduce2q<F>)) != hipSuccess ||
         attr.maxThreadsPerBlock < 8) {
       (void)hipGetLastError();
       return 0u;
@@ -734,8 +768,12 @@ void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* 
       {
         ScopedKTime kt(group == 1 ? "msm_g1_reduce1" : "msm_g2_reduce1", s);
         if (sizeof(typename C::P) > 128) {
-          k_reduce1_serial<F, false><<<blocks_of(ngroups, 256), 256, 0, s>>>(buckets, offsets, p.B, p.m,
-                                                                              (uint32_t)ngroups, lvlS, lvlT, rl1);
+          if (p.quad2)
+            k_reduce1_serialq<F><<<blocks_of(4 * ngroups, 256), 256, 0, s>>>(buckets, offsets, p.B, p.m,
+                                                                             (uint32_t)ngroups, lvlS, lvlT, rl1);
+          else
+            k_reduce1_serial<F, false><<<blocks_of(ngroups, 256), 256, 0, s>>>(buckets, offsets, p.B, p.m,
+                                                                                (uint32_t)ngroups, lvlS, lvlT, rl1);
           k_reduce1_serial<F, true><<<16, 64, 0, s>>>(buckets, offsets, p.B, p.m, (uint32_t)ngroups, lvlS, lvlT,
                                                       rl1);
         } else {
@@ -759,11 +797,9 @@ void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* 
       }
       ScopedKTime kt(group == 1 ? "msm_g1_reduce2" : "msm_g2_reduce2", s);
       bool quad = false;
-      if constexpr (sizeof(typename C::P) <= 128) {
-        if (p.quad2 && p.K <= reduce2_quad_kmax<F>() && p.K >= 2) {
-          k_reduce2q<F><<<nblocks, 4 * p.K, smem, s>>>(lvlS, lvlT, log2m, outPQ, rl2);
-          quad = true;
-        }
+      if (p.quad2 && p.K <= reduce2_quad_kmax<F>() && p.K >= 2) {
+        k_reduce2q<F><<<nblocks, 4 * p.K, smem, s>>>(lvlS, lvlT, log2m, outPQ, rl2);
+        quad = true;
       }
       if (!quad) k_reduce2<F, false><<<nblocks, p.K, smem, s>>>(lvlS, lvlT, log2m, outPQ, rl2);
       k_reduce2<F, true><<<8, p.K, smem, s>>>(lvlS, lvlT, log2m, outPQ, rl2);

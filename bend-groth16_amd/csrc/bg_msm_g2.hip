@@ -10,4 +10,5 @@ void fixed_table_g2(const uint32_t* d_points, uint64_t n, int c, uint32_t* d_out
   fixed_table_curve<Fq2>(d_points, n, c, d_out, s);
 }
 uint64_t accumulate_threads_g2() { return accumulate_round_threads<Fq2>(); }
+uint32_t reduce2_quad_kmax_g2() { return reduce2_quad_kmax<Fq2>(); }
 }  // namespace bg
