@@ -67,14 +67,17 @@ static void finish_plan(MsmPlan& p, int group) {
   // work cancels it)
   uint32_t kmax = group == 1 ? 256 : 128;
   if (group == 2 && p.W == 1 && p.G <= 1024) kmax = 32;
-  // G1's level 2 with a quad per group (k_reduce2q, blocks as wide as that kernel takes)
-  // where the level is latency-bound: at most 2^15 groups (a windowed 2^19 MSM's 69,632
-  // groups fill the device and measured 0.235 -> 0.254 ms with quads) and batches of at
-  // least 2^16 points (smaller proofs replay recorded launch sequences, with which the
-  // 2^14 proof measured 1.30 -> 1.35 ms).  Measured per proof: level-2 time 35-42 % lower
-  // at every size; eager proofs K=10 1.02-1.06 -> 0.98-1.00 ms, K=18 4.01-4.05 -> 3.98-4.01.
+  // The bucket reductions with a quad of lanes per group (k_reduce1q / k_reduce1_serialq,
+  // k_reduce2q; level-2 blocks as wide as that kernel takes) where they are latency-bound:
+  // at most 2^15 level-1 groups (a windowed 2^19 G1 MSM's 69,632 groups fill the device and
+  // its level 2 measured 0.235 -> 0.254 ms with quads).  Measured per proof, alternating
+  // with the one-lane kernels: K=10 0.97-0.98 -> 0.78-0.79 ms, K=12 1.08-1.09 -> 0.92-0.94,
+  // K=14 1.29-1.31 -> 1.14-1.17, K=16 1.85 -> 1.74-1.79, K=18 4.04-4.05 -> 3.97-4.00.
+  // BG16_QUAD_MIN_LOG2 keeps batches below 2^n points on the one-lane kernels.
   const uint32_t kq = group == 1 ? reduce2_quad_kmax_g1() : reduce2_quad_kmax_g2();
-  p.quad2 = kq != 0 && p.n >= (1u << 16) && (uint64_t)p.M * p.W * p.G <= (1u << 15);
+  const int qmin = tune_env("BG16_QUAD_MIN_LOG2");
+  p.quad2 = kq != 0 && (qmin <= 0 || p.n >= (1ull << std::min(qmin, 40))) &&
+            (uint64_t)p.M * p.W * p.G <= (1u << 15);
   if (p.quad2) kmax = std::min(kmax, kq);
   // override: rounded down to a power of two (the level-2 scan and trees need one, and
   // K must divide the group count)
