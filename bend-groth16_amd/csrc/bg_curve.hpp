@@ -252,6 +252,28 @@ struct Quad {
     return {x3, qbc<0>(m), zz3, qbc<1>(m)};
   }
 
+  // Curve::add (exact special cases) on a quad
+  __device__ __forceinline__ static P add(const P& p, const P& q) {
+    if (C::is_inf(p)) return q;
+    if (C::is_inf(q)) return p;
+    const uint32_t r = role();
+    E m = F::mul(qsel(r, p.X, q.X, p.Y, q.Y), qsel(r, q.ZZ, p.ZZ, q.ZZZ, p.ZZZ));
+    const E u1 = qbc<0>(m), u2 = qbc<1>(m), s1 = qbc<2>(m), s2 = qbc<3>(m);
+    const E pp_ = F::sub(u2, u1), rr = F::sub(s2, s1);
+    if (F::is_zero(pp_)) {
+      if (F::is_zero(rr)) return dbl(p);
+      return C::infinity();
+    }
+    m = F::mul(qsel(r, pp_, rr, p.ZZ, p.ZZZ), qsel(r, pp_, rr, q.ZZ, q.ZZZ));
+    const E pp = qbc<0>(m), r2 = qbc<1>(m), zz12 = qbc<2>(m), zzz12 = qbc<3>(m);
+    m = F::mul(qsel(r, pp_, u1, zz12, zz12), pp);
+    const E ppp = qbc<0>(m), q_ = qbc<1>(m), zz3 = qbc<2>(m);
+    const E x3 = F::sub(F::sub(r2, ppp), F::dbl(q_));
+    const E z = F::zero(), t = F::sub(q_, x3);
+    m = F::mul_sub(qsel(r, rr, zzz12, zzz12, zzz12), qsel(r, t, ppp, ppp, ppp), qsel(r, s1, z, z, z), ppp);
+    return {x3, qbc<0>(m), zz3, qbc<1>(m)};
+  }
+
   // Curve::dbl on a quad
   __device__ __forceinline__ static P dbl(const P& p) {
     const uint32_t r = role();

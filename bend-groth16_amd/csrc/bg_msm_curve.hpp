@@ -278,6 +278,103 @@ __global__ __launch_bounds__(MERGE_T) void k_merge_level(const uint32_t* __restr
   }
 }
 
+// k_merge_level with a quad per item (4 MERGE_T threads): the same segments, trees and
+// outputs, each addition spread over the quad's four lanes (Quad); the lanes of a quad
+// hold the same values, its lead lane writes.  For the levels with few blocks (latency-
+// bound); a level that fills the device keeps k_merge_level.
+static constexpr uint32_t MERGE_QT = 128;  // items per block of k_merge_levelq (512 threads)
+
+template <class F, uint32_t MERGE_T = MERGE_QT>
+__global__ __launch_bounds__(4 * MERGE_T) void k_merge_levelq(const uint32_t* __restrict__ ikeys,
+                                                          const uint32_t* __restrict__ iflags,
+                                                          const uint32_t* __restrict__ ipts, uint32_t nitems,
+                                                          uint32_t* __restrict__ buckets, uint32_t* __restrict__ okeys,
+                                                          uint32_t* __restrict__ oflags, uint32_t* __restrict__ opts) {
+  using C = Curve<F>;
+  using Q = Quad<F>;
+  using P = typename C::P;
+  extern __shared__ __align__(16) unsigned char merge_smem[];
+  P* shV = reinterpret_cast<P*>(merge_smem);
+  __shared__ uint32_t shK[MERGE_T], shF[MERGE_T];
+  __shared__ int shS[MERGE_T], shE[MERGE_T];
+  const uint32_t i = threadIdx.x >> 2;  // this quad's item
+  const bool lead = (threadIdx.x & 3u) == 0;
+  const uint32_t b = blockIdx.x;
+  const uint64_t u = (uint64_t)b * MERGE_T + i;
+  const uint64_t xw = C::xyzz_words;
+  uint32_t tk = NOKEY_C, tf = 0;
+  if (u < nitems) {
+    tk = ikeys[2 * u + 1];
+    tf = tk != NOKEY_C ? iflags[2 * u + 1] : 0u;
+  }
+  if (lead) {
+    shK[i] = tk;
+    shF[i] = tf;
+  }
+  P v;
+  if (tk != NOKEY_C) {
+    v = C::load(ipts + xw * (2 * u + 1));
+    if (lead) shV[i] = v;
+  }
+  __syncthreads();
+  bool used = tk != NOKEY_C;
+  int sm = (used && (tf == 0 || i == 0)) ? (int)i : -1;
+  bool cont_next = i + 1 < MERGE_T && shK[i + 1] != NOKEY_C && shF[i + 1] != 0;
+  int em = (used && !cont_next) ? (int)i : (int)MERGE_T;
+  if (lead) {
+    shS[i] = sm;
+    shE[i] = em;
+  }
+  __syncthreads();
+  for (uint32_t d = 1; d < MERGE_T; d <<= 1) {
+    int a = i >= d ? shS[i - d] : -1;
+    int c = i + d < MERGE_T ? shE[i + d] : (int)MERGE_T;
+    __syncthreads();
+    sm = max(sm, a);
+    em = min(em, c);
+    if (lead) {
+      shS[i] = sm;
+      shE[i] = em;
+    }
+    __syncthreads();
+  }
+  const int st = sm, en = em;
+  const uint32_t rel = used ? i - (uint32_t)st : 1u;
+  const uint32_t len = used ? (uint32_t)(en - st + 1) : 0u;
+  for (uint32_t d = 1; d < MERGE_T; d <<= 1) {
+    if (used && (rel & (2 * d - 1)) == 0 && rel + d < len) {
+      v = Q::add(v, shV[i + d]);
+      if (lead) shV[i] = v;
+    }
+    __syncthreads();
+  }
+  if (i == 0 && lead) {
+    uint32_t hk = u < nitems ? ikeys[2 * u] : NOKEY_C;
+    okeys[2 * b] = hk;
+    if (hk != NOKEY_C) C::store(opts + xw * (2 * b), C::load(ipts + xw * (2 * u)));
+  }
+  if (i == MERGE_T - 1 && lead && !used) okeys[2 * b + 1] = NOKEY_C;
+  if (used && rel == 0) {
+    const bool from_prev = i == 0 && tf != 0;
+    if ((uint32_t)en + 1 < MERGE_T) {
+      uint64_t uh = (uint64_t)b * MERGE_T + (uint32_t)en + 1;
+      v = Q::add(v, C::load(ipts + xw * (2 * uh)));
+      if (lead) {
+        if (from_prev) {
+          okeys[2 * b] = tk;
+          C::store(opts + xw * (2 * b), v);
+        } else {
+          C::store(buckets + xw * tk, v);
+        }
+      }
+    } else if (lead) {
+      okeys[2 * b + 1] = tk;
+      oflags[2 * b + 1] = from_prev ? 1u : 0u;
+      C::store(opts + xw * (2 * b + 1), v);
+    }
+  }
+}
+
 // Level 1: group g sums the m = 2^logm buckets i = j*m + l (l < m) of window w:
 //   S = sum_l b_l,  T = sum_l l * b_l.
 // A block of four waves takes 64 groups, one role per wave (roles are wave-uniform, so
@@ -620,6 +717,9 @@ uint32_t reduce2_quad_kmax() {
   return cached;
 }
 
+// merge levels of at most this many blocks run k_merge_levelq
+static constexpr uint32_t MERGE_QUAD_MAX_BLOCKS = 256;
+
 template <class F>
 uint64_t accumulate_round_threads() {
   static uint64_t cached = [] {
@@ -701,8 +801,9 @@ void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* 
   uint32_t* pkeys = w.pkeys.as<uint32_t>(2ull * p.acc_threads);
   uint32_t* pflags = w.pflags.as<uint32_t>(2ull * p.acc_threads);
   uint32_t* ppts = w.ppts.as<uint32_t>(2ull * p.acc_threads * xw);
-  // merge levels ping-pong between two halves of mcap items each
-  const uint64_t mcap = blocks_of(p.acc_threads, MergeT<F>::value);
+  // merge levels ping-pong between two halves of mcap items each (sized for the narrower
+  // of the two merge kernels' blocks)
+  const uint64_t mcap = blocks_of(p.acc_threads, std::min<uint32_t>(MergeT<F>::value, MERGE_QT));
   uint32_t* mkeys = w.mkeys.as<uint32_t>(4 * mcap);
   uint32_t* mflags = w.mflags.as<uint32_t>(4 * mcap);
   uint32_t* mpts = w.mpts.as<uint32_t>(4 * mcap * xw);
@@ -740,11 +841,17 @@ void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* 
         constexpr uint32_t MERGE_T = MergeT<F>::value;
         size_t msmem = MERGE_T * sizeof(typename C::P);
         do {
-          uint32_t blocks = blocks_of(items, MERGE_T);
+          // a level of few blocks is latency-bound: a quad per item (BG16_QUAD=0: never)
+          const bool quad = reduce2_quad_kmax<F>() != 0 && blocks_of(items, MERGE_QT) <= MERGE_QUAD_MAX_BLOCKS;
+          const uint32_t blocks = blocks_of(items, quad ? MERGE_QT : MERGE_T);
           uint32_t* ok = mkeys + (uint64_t)side * 2 * mcap;
           uint32_t* of = mflags + (uint64_t)side * 2 * mcap;
           uint32_t* op = mpts + (uint64_t)side * 2 * mcap * xw;
-          k_merge_level<F><<<blocks, MERGE_T, msmem, s>>>(ik, ifl, ip, items, buckets, ok, of, op);
+          if (quad)
+            k_merge_levelq<F><<<blocks, 4 * MERGE_QT, MERGE_QT * sizeof(typename C::P), s>>>(ik, ifl, ip, items,
+                                                                                          buckets, ok, of, op);
+          else
+            k_merge_level<F><<<blocks, MERGE_T, msmem, s>>>(ik, ifl, ip, items, buckets, ok, of, op);
           ik = ok;
           ifl = of;
           ip = op;
