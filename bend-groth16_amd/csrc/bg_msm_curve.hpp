@@ -152,13 +152,20 @@ __global__ __launch_bounds__(256, 2) void k_accumulate(const uint64_t* __restric
 }
 
 // Exact recomputation of the flagged buckets (doubling / cancellation inside a bucket:
-// repeated or opposite bases), with the special-case-complete mixed add.
+// repeated or opposite bases), with the special-case-complete mixed add.  It also clears
+// the reductions' exception counts rl1[0] and rl2[0] (the reductions run next on the same
+// stream), which saves two launches per MSM.
 template <class F>
 __global__ __launch_bounds__(64) void k_fixup(const uint64_t* __restrict__ ptrs, uint32_t kspan,
                                               const uint2* __restrict__ sorted,
                                               const uint32_t* __restrict__ offsets, uint32_t nb,
-                                              const uint32_t* __restrict__ fix, uint32_t* __restrict__ buckets) {
+                                              const uint32_t* __restrict__ fix, uint32_t* __restrict__ buckets,
+                                              uint32_t* __restrict__ rl1, uint32_t* __restrict__ rl2) {
   using C = Curve<F>;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    rl1[0] = 0;
+    rl2[0] = 0;
+  }
   uint32_t count = fix[0];
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
     uint32_t key = fix[1 + nb + i];
@@ -859,7 +866,7 @@ void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* 
           side ^= 1;
         } while (items > 1);
       }
-      k_fixup<F><<<4, 64, 0, s>>>(dptrs, p.kspan, sorted, offsets, p.nb, fix, buckets);
+      k_fixup<F><<<4, 64, 0, s>>>(dptrs, p.kspan, sorted, offsets, p.nb, fix, buckets, rl1, rl2);
       BG_HIP(hipGetLastError());
     }
     if (!w.mid) BG_HIP(hipEventCreateWithFlags(&w.mid, hipEventDisableTiming));
@@ -870,8 +877,6 @@ void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* 
       unsigned nblocks = (unsigned)(ngroups / p.K);
       size_t smem = 2 * (size_t)p.K * sizeof(typename C::P);
       BG_REQUIRE(smem <= 65536, BG_ERR_INTERNAL, "msm: level-2 block %u needs %zu B of LDS", p.K, smem);
-      launch_zero_u32(rl1, 1, s);
-      launch_zero_u32(rl2, 1, s);
       {
         ScopedKTime kt(group == 1 ? "msm_g1_reduce1" : "msm_g2_reduce1", s);
         if (sizeof(typename C::P) > 128) {
