@@ -708,9 +708,9 @@ __global__ __launch_bounds__(QuadThreads<F>::value) void k_reduce2q(const uint32
 // threads as its registers allow in one block (0 when BG16_QUAD=0).
 template <class F>
 uint32_t reduce2_quad_kmax() {
+  const char* e = getenv("BG16_QUAD");  // read on every plan, so a process can switch
+  if (e && atoi(e) == 0) return 0u;
   static const uint32_t cached = [] {
-    const char* e = getenv("BG16_QUAD");
-    if (e && atoi(e) == 0) return 0u;
     hipFuncAttributes attr;
     if (hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(&k_reduce2q<F>)) != hipSuccess ||
         attr.maxThreadsPerBlock < 8) {
