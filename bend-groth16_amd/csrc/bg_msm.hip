@@ -73,11 +73,14 @@ static void finish_plan(MsmPlan& p, int group) {
   // its level 2 measured 0.235 -> 0.254 ms with quads).  Measured per proof, alternating
   // with the one-lane kernels: K=10 0.97-0.98 -> 0.78-0.79 ms, K=12 1.08-1.09 -> 0.92-0.94,
   // K=14 1.29-1.31 -> 1.14-1.17, K=16 1.85 -> 1.74-1.79, K=18 4.04-4.05 -> 3.97-4.00.
-  // BG16_QUAD_MIN_LOG2 keeps batches below 2^n points on the one-lane kernels.
+  // BG16_QUAD_MIN_LOG2 keeps batches below 2^n points on the one-lane kernels;
+  // BG16_QUAD_GROUPS_LOG2 / BG16_QUAD_GROUPS_LOG2_G2 move the group-count limit.
   const uint32_t kq = group == 1 ? reduce2_quad_kmax_g1() : reduce2_quad_kmax_g2();
   const int qmin = tune_env("BG16_QUAD_MIN_LOG2");
+  int qgroups = tune_env(group == 1 ? "BG16_QUAD_GROUPS_LOG2" : "BG16_QUAD_GROUPS_LOG2_G2");
+  if (qgroups <= 0) qgroups = 15;
   p.quad2 = kq != 0 && (qmin <= 0 || p.n >= (1ull << std::min(qmin, 40))) &&
-            (uint64_t)p.M * p.W * p.G <= (1u << 15);
+            (uint64_t)p.M * p.W * p.G <= (1ull << std::min(qgroups, 40));
   if (p.quad2) kmax = std::min(kmax, kq);
   // override: rounded down to a power of two (the level-2 scan and trees need one, and
   // K must divide the group count)

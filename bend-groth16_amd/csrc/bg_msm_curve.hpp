@@ -68,6 +68,22 @@ __device__ __forceinline__ void load_vec(const uint32_t* __restrict__ p, uint32_
     w[4 * i + 3] = v.w;
   }
 }
+// The same load through a global-address-space pointer (bases reached through the pointer
+// table are generic pointers, and flat loads make every later wait a wait for all loads).
+template <int W>
+__device__ __forceinline__ void load_vec_global(const uint32_t* p, uint32_t* w) {
+  typedef uint32_t U4 __attribute__((ext_vector_type(4)));
+  typedef const __attribute__((address_space(1))) U4 GlobalU4;
+  const GlobalU4* q = (const GlobalU4*)p;
+#pragma unroll
+  for (int i = 0; i < W / 4; i++) {
+    const U4 v = q[i];
+    w[4 * i] = v.x;
+    w[4 * i + 1] = v.y;
+    w[4 * i + 2] = v.z;
+    w[4 * i + 3] = v.w;
+  }
+}
 template <int W>
 __device__ __forceinline__ void store_vec(uint32_t* __restrict__ p, const uint32_t* w) {
   uint4* q = reinterpret_cast<uint4*>(p);
@@ -111,15 +127,24 @@ __global__ __launch_bounds__(256, 2) void k_accumulate(const uint64_t* __restric
   // One point load, one run start, one addition and one store site in the loop: the
   // digit-domain conversions at a run's start and end are whole products, so each is
   // emitted once (a smaller kernel for the instruction cache).  Step e == end closes the
-  // last run.
+  // last run.  Loads run ahead of the additions: the entry two steps ahead and (G1, whose
+  // 16-word point fits beside the accumulator within three waves per SIMD) the next
+  // point's words are in flight while the current addition runs.  Their addresses stay in
+  // bounds on every step (the last entry's index, point 0 of the current MSM), so the
+  // loads are unconditional and every wait is for the oldest loads only.
+  constexpr bool PF = A::aff_words <= 16;
+  uint2 ent = sorted[start];                                                 // entry e
+  uint2 ent1 = start + 1 < end ? sorted[start + 1] : make_uint2(NOKEY_C, 0u);  // entry e + 1
+  const uint32_t* points = entry_points(ptrs, ent.x, kspan);  // base of entry e's MSM
+  uint64_t kend = ((uint64_t)(ent.x / kspan) + 1) * kspan;   // end of that MSM's keys
+  uint32_t w[A::aff_words];
+  if constexpr (PF) load_vec_global<A::aff_words>(points + (uint64_t)A::aff_words * (ent.y & 0x7FFFFFFFu), w);
   uint32_t cur = NOKEY_C;
-  const uint32_t* points = nullptr;
   bool first = true;
   uint32_t bad = 0;
   typename A::Pt acc;
   for (uint64_t e = start;; e++) {
     const bool last = e == end;
-    const uint2 ent = last ? make_uint2(NOKEY_C, 0u) : sorted[e];
     if (e > start && ent.x != cur) {
       // the run of key cur ends: a complete bucket, or a partial for the merge
       const bool cp = first && prevkey == cur;
@@ -138,8 +163,21 @@ __global__ __launch_bounds__(256, 2) void k_accumulate(const uint64_t* __restric
       first = false;
     }
     if (last) break;
-    if (e == start || ent.x / kspan != cur / kspan) points = entry_points(ptrs, ent.x, kspan);
-    const typename A::Raw pt = load_point<A>(points + (uint64_t)A::aff_words * (ent.y & 0x7FFFFFFFu));
+    const bool more = e + 1 < end;
+    typename A::Raw pt;
+    if constexpr (!PF) pt = load_point<A>(points + (uint64_t)A::aff_words * (ent.y & 0x7FFFFFFFu));
+    // entry e + 1's MSM base (rarely a new one) before the entry two ahead is requested
+    // (keys ascend along the slice, so a key at or past kend belongs to a later MSM)
+    if (more && ent1.x >= kend) {
+      points = entry_points(ptrs, ent1.x, kspan);
+      kend = ((uint64_t)(ent1.x / kspan) + 1) * kspan;
+    }
+    uint2 ent2 = sorted[e + 2 < end ? e + 2 : end - 1];
+    if (e + 2 >= end) ent2 = make_uint2(NOKEY_C, 0u);
+    if constexpr (PF) {
+      pt = A::load(w);
+      load_vec_global<A::aff_words>(points + (uint64_t)A::aff_words * (ent1.y & 0x7FFFFFFFu), w);
+    }
     const bool neg = (ent.y >> 31) != 0;
     if (ent.x != cur) {
       cur = ent.x;
@@ -148,6 +186,8 @@ __global__ __launch_bounds__(256, 2) void k_accumulate(const uint64_t* __restric
     } else {
       acc = A::madd(acc, pt, neg, bad);
     }
+    ent = ent1;
+    ent1 = ent2;
   }
 }
 
