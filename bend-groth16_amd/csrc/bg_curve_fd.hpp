@@ -65,6 +65,8 @@ struct G1Acc {
     to_words<P>(p.ZZ, w + 16);
     to_words<P>(p.ZZZ, w + 24);
   }
+  // one digit-form coordinate (the k-th Fd of Pt) to its 8 words (words 8k .. 8k + 7 of store)
+  BG_HD static void store_fd(uint32_t* w, const Fd& a) { to_words<P>(a, w); }
 };
 
 template <class P>
@@ -121,6 +123,8 @@ struct G2Acc {
     const Fd y1s[4] = {t1, t0, nppp1, nppp0};
     return {x3, {mul4<P>(xs, y0s), mul4<P>(xs, y1s)}, zz3, zzz3};
   }
+  // one digit-form coordinate (the k-th Fd of Pt) to its 8 words (words 8k .. 8k + 7 of store)
+  BG_HD static void store_fd(uint32_t* w, const Fd& a) { to_words<P>(a, w); }
   BG_HD static void store(uint32_t* w, const Pt& p) {
     const Fd2* c[4] = {&p.X, &p.Y, &p.ZZ, &p.ZZZ};
 #pragma unroll

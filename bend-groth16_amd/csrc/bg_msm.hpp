@@ -48,7 +48,7 @@ int fixed_window_bits(const uint64_t* ns, uint32_t M, int group);
 // Work buffers of one in-flight MSM (reused across calls).
 struct MsmWork {
   DevBuf keys, vals, ranks, counts, sorted, buckets, pkeys, pflags, ppts, mkeys, mflags, mpts, lvlS, lvlT, outPQ,
-      scan_tmp, fix, rfix, ptrs, scal, hist, mq;
+      scan_tmp, fix, rfix, ptrs, scal, hist, mq, stage;
   HostBuf host_out, host_ptrs;
   // Coherent pinned landing of the outputs of a recorded MSM (no event nodes in a graph):
   // PUB_HEAD words whose first is a flag the device raises once the outputs follow it.
@@ -69,13 +69,13 @@ struct MsmWork {
   uint64_t bytes() const {
     uint64_t t = 0;
     for (const DevBuf* b : {&keys, &vals, &ranks, &counts, &sorted, &buckets, &pkeys, &pflags, &ppts, &mkeys, &mflags,
-                            &mpts, &lvlS, &lvlT, &outPQ, &scan_tmp, &fix, &rfix, &ptrs, &scal, &hist, &mq})
+                            &mpts, &lvlS, &lvlT, &outPQ, &scan_tmp, &fix, &rfix, &ptrs, &scal, &hist, &mq, &stage})
       t += b->cap;
     return t;
   }
   void release() {
     for (DevBuf* b : {&keys, &vals, &ranks, &counts, &sorted, &buckets, &pkeys, &pflags, &ppts, &mkeys, &mflags,
-                      &mpts, &lvlS, &lvlT, &outPQ, &scan_tmp, &fix, &rfix, &ptrs, &scal, &hist, &mq})
+                      &mpts, &lvlS, &lvlT, &outPQ, &scan_tmp, &fix, &rfix, &ptrs, &scal, &hist, &mq, &stage})
       b->release();
     host_out.release();
     host_ptrs.release();

@@ -103,14 +103,73 @@ __device__ __forceinline__ void store_acc(uint32_t* __restrict__ p, const typena
   store_vec<A::xyzz_words>(p, w);
 }
 
+// A closed run's digit-form sum waits in a per-thread staging record until a wave-wide
+// write-out converts it to the word domain (the conversion then runs once for every lane
+// with a waiting run, instead of once per step at which any lane of the wave closes one).
+// Record of thread t, coordinate k (the k-th digit vector of A::Pt): digits 0..3 and 4..7
+// as 16-byte chunks 2k and 2k + 1 of column t, digit 8 in a word array after the chunks,
+// then the run's key and destination kind (a word pair), so a wave's lanes write and read
+// whole contiguous spans and nothing of the waiting run is held in registers.
+template <class A>
+struct StageOf {
+  static constexpr uint32_t coords = sizeof(typename A::Pt) / sizeof(fd::Fd);
+  static constexpr uint64_t bytes_per_thread = 36ull * coords + 8;
+  static_assert(sizeof(typename A::Pt) == coords * sizeof(fd::Fd), "digit-form point layout");
+};
+template <class A>
+__device__ __forceinline__ void stage_put(uint32_t* __restrict__ st, uint32_t nthreads, uint32_t t,
+                                          const typename A::Pt& p, uint32_t key, uint32_t kind) {
+  constexpr uint32_t NC = StageOf<A>::coords;
+  fd::Fd c[NC];
+  __builtin_memcpy(c, &p, sizeof p);
+  uint4* q = reinterpret_cast<uint4*>(st);
+  uint32_t* d8 = st + 8ull * NC * nthreads;
+#pragma unroll
+  for (uint32_t k = 0; k < NC; k++) {
+    q[(uint64_t)(2 * k) * nthreads + t] = make_uint4(c[k].d[0], c[k].d[1], c[k].d[2], c[k].d[3]);
+    q[(uint64_t)(2 * k + 1) * nthreads + t] = make_uint4(c[k].d[4], c[k].d[5], c[k].d[6], c[k].d[7]);
+    d8[(uint64_t)k * nthreads + t] = c[k].d[8];
+  }
+  reinterpret_cast<uint2*>(st + 9ull * NC * nthreads)[t] = make_uint2(key, kind);
+}
+template <class A>
+__device__ __forceinline__ uint2 stage_meta(const uint32_t* __restrict__ st, uint32_t nthreads, uint32_t t) {
+  return reinterpret_cast<const uint2*>(st + 9ull * StageOf<A>::coords * nthreads)[t];
+}
+// thread t's staged run, converted coordinate by coordinate, to the XYZZ words at dst
+template <class A>
+__device__ __forceinline__ void stage_write(const uint32_t* __restrict__ st, uint32_t nthreads, uint32_t t,
+                                            uint32_t* __restrict__ dst) {
+  constexpr uint32_t NC = StageOf<A>::coords;
+  const uint4* q = reinterpret_cast<const uint4*>(st);
+  const uint32_t* d8 = st + 8ull * NC * nthreads;
+#pragma unroll
+  for (uint32_t k = 0; k < NC; k++) {
+    const uint4 a = q[(uint64_t)(2 * k) * nthreads + t];
+    const uint4 b = q[(uint64_t)(2 * k + 1) * nthreads + t];
+    const fd::Fd f = {{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, d8[(uint64_t)k * nthreads + t]}};
+    uint32_t w[8];
+    A::store_fd(w, f);
+    store_vec<8>(dst + 8 * k, w);
+    // one coordinate's registers at a time (the write-out runs beside the live accumulator)
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// G1 accumulation keeps three waves per SIMD (at most 168 VGPRs), G2 two
 template <class F>
-__global__ __launch_bounds__(256, 2) void k_accumulate(const uint64_t* __restrict__ ptrs, uint32_t kspan,
+struct AccWaves {
+  static constexpr int value = sizeof(typename Curve<F>::P) > 128 ? 2 : 3;
+};
+
+template <class F>
+__global__ __launch_bounds__(256, AccWaves<F>::value) void k_accumulate(const uint64_t* __restrict__ ptrs, uint32_t kspan,
                                                     const uint2* __restrict__ sorted,
                                                     const uint32_t* __restrict__ total_ptr, uint32_t L,
                                                     uint32_t nthreads, uint32_t* __restrict__ buckets,
                                                     uint32_t* __restrict__ pkeys, uint32_t* __restrict__ pflags,
                                                     uint32_t* __restrict__ ppts, uint32_t nb,
-                                                    uint32_t* __restrict__ fix) {
+                                                    uint32_t* __restrict__ fix, uint32_t* __restrict__ stage) {
   using C = Curve<F>;
   using A = typename AccOf<F>::type;
   static_assert(A::aff_words == C::aff_words && A::xyzz_words == C::xyzz_words, "point layout");
@@ -118,10 +177,12 @@ __global__ __launch_bounds__(256, 2) void k_accumulate(const uint64_t* __restric
   if (t >= nthreads) return;
   pkeys[2 * t] = NOKEY_C;
   pkeys[2 * t + 1] = NOKEY_C;
-  uint32_t T = *total_ptr;
-  uint64_t start = (uint64_t)t * L;
-  if (start >= T) return;
-  uint64_t end = min(start + L, (uint64_t)T);
+  // entry indices fit in 32 bits (the planner keeps a batch below 2^32 - 1 entries)
+  const uint32_t T = *total_ptr;
+  const uint64_t start64 = (uint64_t)t * L;
+  if (start64 >= T) return;
+  const uint32_t start = (uint32_t)start64;
+  const uint32_t end = (uint32_t)min(start64 + L, (uint64_t)T);
   uint32_t prevkey = start > 0 ? sorted[start - 1].x : NOKEY_C;
   uint32_t nextkey = end < T ? sorted[end].x : NOKEY_C;
   // One point load, one run start, one addition and one store site in the loop: the
@@ -134,49 +195,97 @@ __global__ __launch_bounds__(256, 2) void k_accumulate(const uint64_t* __restric
   // loads are unconditional and every wait is for the oldest loads only.
   constexpr bool PF = A::aff_words <= 16;
   uint2 ent = sorted[start];                                                 // entry e
-  uint2 ent1 = start + 1 < end ? sorted[start + 1] : make_uint2(NOKEY_C, 0u);  // entry e + 1
+  uint2 ent1 = start + 1 < end ? sorted[start + 1] : make_uint2(NOKEY_C, 0u);  // entry e + 1 (G1)
   const uint32_t* points = entry_points(ptrs, ent.x, kspan);  // base of entry e's MSM
-  uint64_t kend = ((uint64_t)(ent.x / kspan) + 1) * kspan;   // end of that MSM's keys
+  uint32_t kend = (ent.x / kspan + 1) * kspan;               // end of that MSM's keys
   uint32_t w[A::aff_words];
   if constexpr (PF) load_vec_global<A::aff_words>(points + (uint64_t)A::aff_words * (ent.y & 0x7FFFFFFFu), w);
   uint32_t cur = NOKEY_C;
   bool first = true;
   uint32_t bad = 0;
   typename A::Pt acc;
-  for (uint64_t e = start;; e++) {
+  // a run waits in the staging record (with its key and where it goes -- SK_BUCKET: a
+  // complete bucket; SK_HEAD: head slot 2t, flag 1; SK_TAIL | flag: tail slot 2t + 1)
+  constexpr uint32_t SK_BUCKET = 1, SK_HEAD = 2, SK_TAIL = 4;
+  bool staged = false;
+  // Step e == end closes the last run; step end + 1 writes out what is still staged.
+  // (G2 converts and writes each run where it closes: its addition is 2.8 times G1's, so
+  // the conversion is a smaller share of a step, and staging costs it registers it lacks.)
+  constexpr bool STAGE = PF;
+  for (uint32_t e = start;; e++) {
+    const bool drain = e > end;
     const bool last = e == end;
-    if (e > start && ent.x != cur) {
+    const bool closes = !drain && e > start && ent.x != cur;
+    // a lane about to stage a run while one waits: the wave writes its staged runs out
+    if (STAGE && __any((closes || drain) && staged)) {
+      if (staged) {
+        const uint2 meta = stage_meta<A>(stage, nthreads, t);  // key, kind
+        uint32_t* dst;
+        if (meta.y == SK_BUCKET) {
+          dst = buckets + (uint64_t)A::xyzz_words * meta.x;
+        } else {
+          const uint32_t slot = meta.y == SK_HEAD ? 2 * t : 2 * t + 1;
+          pkeys[slot] = meta.x;
+          pflags[slot] = meta.y == SK_HEAD ? 1u : meta.y & 1u;
+          dst = ppts + (uint64_t)A::xyzz_words * slot;
+        }
+        stage_write<A>(stage, nthreads, t, dst);
+        staged = false;
+      }
+    }
+    if (drain) break;
+    if (closes) {
       // the run of key cur ends: a complete bucket, or a partial for the merge
       const bool cp = first && prevkey == cur;
       const bool cn = last && nextkey == cur;
       if (bad) flag_bucket<F>(cur, nb, fix);
-      uint32_t* dst;
-      if (!cp && !cn) {
-        dst = buckets + (uint64_t)A::xyzz_words * cur;
+      const uint32_t slot = !cp && !cn ? NOKEY_C : (cn ? 2 * t + 1 : 2 * t);
+      if constexpr (STAGE) {
+        stage_put<A>(stage, nthreads, t, acc, cur,
+                     slot == NOKEY_C ? SK_BUCKET : (cn ? SK_TAIL | (cp ? 1u : 0u) : SK_HEAD));
+        staged = true;
       } else {
-        const uint32_t slot = cn ? 2 * t + 1 : 2 * t;
-        pkeys[slot] = cur;
-        pflags[slot] = cp ? 1u : 0u;
-        dst = ppts + (uint64_t)A::xyzz_words * slot;
+        uint32_t* dst;
+        if (slot == NOKEY_C) {
+          dst = buckets + (uint64_t)A::xyzz_words * cur;
+        } else {
+          pkeys[slot] = cur;
+          pflags[slot] = cp ? 1u : 0u;
+          dst = ppts + (uint64_t)A::xyzz_words * slot;
+        }
+        store_acc<A>(dst, acc);
       }
-      store_acc<A>(dst, acc);
       first = false;
     }
-    if (last) break;
+    if (last) {
+      if constexpr (STAGE) continue;
+      break;
+    }
     const bool more = e + 1 < end;
     typename A::Raw pt;
-    if constexpr (!PF) pt = load_point<A>(points + (uint64_t)A::aff_words * (ent.y & 0x7FFFFFFFu));
-    // entry e + 1's MSM base (rarely a new one) before the entry two ahead is requested
-    // (keys ascend along the slice, so a key at or past kend belongs to a later MSM)
-    if (more && ent1.x >= kend) {
-      points = entry_points(ptrs, ent1.x, kspan);
-      kend = ((uint64_t)(ent1.x / kspan) + 1) * kspan;
-    }
-    uint2 ent2 = sorted[e + 2 < end ? e + 2 : end - 1];
-    if (e + 2 >= end) ent2 = make_uint2(NOKEY_C, 0u);
+    uint2 nxt;  // entry e + 1
     if constexpr (PF) {
+      // entry e + 1's MSM base (rarely a new one) before the entry two ahead is requested
+      // (keys ascend along the slice, so a key at or past kend belongs to a later MSM)
+      if (more && ent1.x >= kend) {
+        points = entry_points(ptrs, ent1.x, kspan);
+        kend = (ent1.x / kspan + 1) * kspan;
+      }
+      uint2 ent2 = sorted[e + 2 < end ? e + 2 : end - 1];
+      if (e + 2 >= end) ent2 = make_uint2(NOKEY_C, 0u);
       pt = A::load(w);
       load_vec_global<A::aff_words>(points + (uint64_t)A::aff_words * (ent1.y & 0x7FFFFFFFu), w);
+      nxt = ent1;
+      ent1 = ent2;
+    } else {
+      // G2: the next entry one step ahead, this entry's point loaded here
+      if (ent.x >= kend) {
+        points = entry_points(ptrs, ent.x, kspan);
+        kend = (ent.x / kspan + 1) * kspan;
+      }
+      pt = load_point<A>(points + (uint64_t)A::aff_words * (ent.y & 0x7FFFFFFFu));
+      nxt = sorted[more ? e + 1 : e];
+      if (!more) nxt = make_uint2(NOKEY_C, 0u);
     }
     const bool neg = (ent.y >> 31) != 0;
     if (ent.x != cur) {
@@ -186,8 +295,7 @@ __global__ __launch_bounds__(256, 2) void k_accumulate(const uint64_t* __restric
     } else {
       acc = A::madd(acc, pt, neg, bad);
     }
-    ent = ent1;
-    ent1 = ent2;
+    ent = nxt;
   }
 }
 
@@ -830,7 +938,7 @@ void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* 
       BG_REQUIRE((uint64_t)p.Wd * ns[i] < (1ull << 31), BG_ERR_ARG, "msm: fixed-base table of %llu points too large",
                  (unsigned long long)ns[i]);
   const uint64_t n = p.n;
-  BG_REQUIRE(p.entries < (1ull << 32) && (uint64_t)M * p.kspan < (1ull << 31), BG_ERR_ARG,
+  BG_REQUIRE(p.entries < (1ull << 32) - 1 && (uint64_t)M * p.kspan < (1ull << 31), BG_ERR_ARG,
              "msm: batch too large (%llu entries)", (unsigned long long)p.entries);
   BG_REQUIRE(p.c >= 2 && p.c <= 22, BG_ERR_ARG, "msm: window_bits %d out of range", p.c);
   w.plan = p;
@@ -848,6 +956,8 @@ void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* 
   uint32_t* pkeys = w.pkeys.as<uint32_t>(2ull * p.acc_threads);
   uint32_t* pflags = w.pflags.as<uint32_t>(2ull * p.acc_threads);
   uint32_t* ppts = w.ppts.as<uint32_t>(2ull * p.acc_threads * xw);
+  uint32_t* stage = static_cast<uint32_t*>(
+      w.stage.get((uint64_t)p.acc_threads * StageOf<typename AccOf<F>::type>::bytes_per_thread));
   // merge levels ping-pong between two halves of mcap items each (sized for the narrower
   // of the two merge kernels' blocks)
   const uint64_t mcap = blocks_of(p.acc_threads, std::min<uint32_t>(MergeT<F>::value, MERGE_QT));
@@ -876,7 +986,7 @@ void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* 
         ScopedKTime kt(group == 1 ? "msm_g1_accumulate" : "msm_g2_accumulate", s);
         k_accumulate<F><<<blocks_of(p.acc_threads, 256), 256, 0, s>>>(dptrs, p.kspan, sorted, offsets + p.nb, p.L,
                                                                        p.acc_threads, buckets, pkeys, pflags, ppts,
-                                                                       p.nb, fix);
+                                                                       p.nb, fix, stage);
       }
       ScopedKTime kt(group == 1 ? "msm_g1_merge" : "msm_g2_merge", s);
       {
