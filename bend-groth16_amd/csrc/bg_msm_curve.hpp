@@ -186,13 +186,14 @@ __global__ __launch_bounds__(256, AccWaves<F>::value) void k_accumulate(const ui
   uint32_t prevkey = start > 0 ? sorted[start - 1].x : NOKEY_C;
   uint32_t nextkey = end < T ? sorted[end].x : NOKEY_C;
   // One point load, one run start, one addition and one store site in the loop: the
-  // digit-domain conversions at a run's start and end are whole products, so each is
-  // emitted once (a smaller kernel for the instruction cache).  Step e == end closes the
-  // last run.  Loads run ahead of the additions: the entry two steps ahead and (G1, whose
-  // 16-word point fits beside the accumulator within three waves per SIMD) the next
-  // point's words are in flight while the current addition runs.  Their addresses stay in
-  // bounds on every step (the last entry's index, point 0 of the current MSM), so the
-  // loads are unconditional and every wait is for the oldest loads only.
+  // conversions at a run's start (two reductions into the digit domain) and at its write
+  // (four packings into words) are long, so each is emitted once (a smaller kernel for the
+  // instruction cache).  Step e == end closes the last run.  Loads run ahead of the
+  // additions: G1 (whose 16-word point fits beside the accumulator within three waves per
+  // SIMD) requests the entry two steps ahead and the next point's words before the current
+  // addition, at addresses kept in bounds on every step (the last entry's index, point 0
+  // of the current MSM), so the loads are unconditional and every wait is for the oldest
+  // loads only; G2 requests the next entry.
   constexpr bool PF = A::aff_words <= 16;
   uint2 ent = sorted[start];                                                 // entry e
   uint2 ent1 = start + 1 < end ? sorted[start + 1] : make_uint2(NOKEY_C, 0u);  // entry e + 1 (G1)
