@@ -10,6 +10,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <array>
 #include "bg_runtime.hpp"
 
 namespace bg {
@@ -66,17 +67,24 @@ struct MsmWork {
   int group = 1;
   hipEvent_t done = nullptr;
   hipEvent_t mid = nullptr;  // recorded once the buckets are complete (before the reductions)
+  // every device buffer: the byte count, the release and the buffer signature of recorded
+  // launch sequences all walk this one list
+  template <class Self>
+  static auto buf_list(Self& w) {
+    return std::array<decltype(&w.keys), 23>{&w.keys, &w.vals, &w.ranks, &w.counts, &w.sorted, &w.buckets,
+                                             &w.pkeys, &w.pflags, &w.ppts, &w.mkeys, &w.mflags, &w.mpts,
+                                             &w.lvlS, &w.lvlT, &w.outPQ, &w.scan_tmp, &w.fix, &w.rfix,
+                                             &w.ptrs, &w.scal, &w.hist, &w.mq, &w.stage};
+  }
+  auto bufs() const { return buf_list(*this); }
+  auto bufs() { return buf_list(*this); }
   uint64_t bytes() const {
     uint64_t t = 0;
-    for (const DevBuf* b : {&keys, &vals, &ranks, &counts, &sorted, &buckets, &pkeys, &pflags, &ppts, &mkeys, &mflags,
-                            &mpts, &lvlS, &lvlT, &outPQ, &scan_tmp, &fix, &rfix, &ptrs, &scal, &hist, &mq, &stage})
-      t += b->cap;
+    for (const DevBuf* b : bufs()) t += b->cap;
     return t;
   }
   void release() {
-    for (DevBuf* b : {&keys, &vals, &ranks, &counts, &sorted, &buckets, &pkeys, &pflags, &ppts, &mkeys, &mflags,
-                      &mpts, &lvlS, &lvlT, &outPQ, &scan_tmp, &fix, &rfix, &ptrs, &scal, &hist, &mq, &stage})
-      b->release();
+    for (DevBuf* b : bufs()) b->release();
     host_out.release();
     host_ptrs.release();
     host_pub.release();

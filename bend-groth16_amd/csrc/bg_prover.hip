@@ -476,10 +476,7 @@ uint64_t ProveSlot::buffer_signature() const {
   uint64_t sig = 0x6a09e667f3bcc909ULL;
   for (const DevArray* a : {&z, &zc, &ea, &eb, &ec, &h, &tmp}) sig = mix64(mix64(sig, (uint64_t)a->ptr), a->bytes);
   for (const MsmWork& w : work) {
-    for (const DevBuf* b : {&w.keys, &w.vals, &w.ranks, &w.counts, &w.sorted, &w.buckets, &w.pkeys, &w.pflags,
-                            &w.ppts, &w.mkeys, &w.mflags, &w.mpts, &w.lvlS, &w.lvlT, &w.outPQ, &w.scan_tmp, &w.fix,
-                            &w.rfix, &w.ptrs, &w.scal, &w.hist, &w.mq})
-      sig = mix64(mix64(sig, (uint64_t)b->ptr), b->cap);
+    for (const DevBuf* b : w.bufs()) sig = mix64(mix64(sig, (uint64_t)b->ptr), b->cap);
     for (const HostBuf* b : {&w.host_out, &w.host_ptrs, &w.host_pub}) sig = mix64(mix64(sig, (uint64_t)b->ptr), b->cap);
     sig = mix64(mix64(sig, (uint64_t)w.done), (uint64_t)w.mid);
   }

@@ -121,9 +121,11 @@ def test_graph_device_witness(bg, gkeys, monkeypatch):
     pk.close()
 
 
-def test_engine_slot_alternating_keys(bg, gkeys, monkeypatch):
+@pytest.mark.parametrize("order", [(12, 4, 10), (4, 10, 12)], ids=["large-first", "growing"])
+def test_engine_slot_alternating_keys(bg, gkeys, monkeypatch, order):
     """keys of three sizes take turns on one engine slot: each keeps its own graph and
-    every proof equals the eager proof made alone"""
+    every proof equals the eager proof made alone (growing: the slot's work buffers are
+    reallocated after smaller keys ran, so sequences recorded before must not be replayed)"""
     pks = {lg: bendg.ProvingKey(d) for lg, d in gkeys.items()}
     files = {lg: (np.fromfile(d / "witness.bin", dtype=np.uint32), np.fromfile(d / "rs.bin", dtype=np.uint32))
              for lg, d in gkeys.items()}
@@ -134,7 +136,7 @@ def test_engine_slot_alternating_keys(bg, gkeys, monkeypatch):
     eng = bendg.Engine(slots=1)
     try:
         for rnd in range(4):
-            for lg in (12, 4, 10):
+            for lg in order:
                 print(f"engine round {rnd} K={lg}", flush=True)
                 t = eng.submit(pks[lg], *files[lg])
                 proof, _ = eng.wait(t)
