@@ -441,6 +441,37 @@ __global__ void k_scan_add(uint32_t* __restrict__ out, const uint32_t* __restric
   if (i < n) out[i] += bsum[i / (2 * SCAN_T)];
 }
 
+// k_scan_top and k_scan_add in one launch for at most SCAN_FOLD_MAX_BLOCKS local blocks:
+// block b adds up the block sums before its own (at most 4 reads a thread) and adds them
+// to its 2 SCAN_T elements; the last block writes the total.
+static constexpr uint32_t SCAN_FOLD_MAX_BLOCKS = 4 * SCAN_T;
+__global__ __launch_bounds__(SCAN_T) void k_scan_add_fold(uint32_t* __restrict__ out,
+                                                          const uint32_t* __restrict__ bsum, uint32_t n,
+                                                          uint32_t* __restrict__ total) {
+  __shared__ uint32_t sh[SCAN_T];
+  const uint32_t b = blockIdx.x, t = threadIdx.x;
+  uint32_t sum = 0;
+  for (uint32_t j = t; j < b; j += SCAN_T) sum += bsum[j];
+  sh[t] = sum;
+  __syncthreads();
+  for (uint32_t d = SCAN_T / 2; d > 0; d >>= 1) {
+    if (t < d) sh[t] += sh[t + d];
+    __syncthreads();
+  }
+  const uint32_t pre = sh[0];
+  const uint64_t i0 = (uint64_t)b * 2 * SCAN_T + t, i1 = i0 + SCAN_T;
+  if (i0 < n) out[i0] += pre;
+  if (i1 < n) out[i1] += pre;
+  if (b + 1 == gridDim.x && t == 0) *total = pre + bsum[b];
+}
+
+// zeroes two word ranges in one launch
+__global__ void k_zero2_u32(uint32_t* __restrict__ a, uint64_t na, uint32_t* __restrict__ b, uint64_t nb) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < na) a[i] = 0;
+  else if (i - na < nb) b[i - na] = 0;
+}
+
 __global__ void k_copy_u32(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = in[i];
@@ -817,18 +848,30 @@ uint64_t msm_hist_words(const MsmPlan& p, const uint64_t* ns) {
   return std::max<uint64_t>(t, 1);
 }
 
+// exclusive scan of in (n words) into out (in place when out == in); *total = the sum
+static void scan_excl(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* bsum, uint32_t* total,
+                      hipStream_t s) {
+  const uint32_t nblk = msm_scan_blocks((uint32_t)n);
+  k_scan_local<<<nblk, SCAN_T, 0, s>>>(in, out, bsum, (uint32_t)n);
+  if (nblk <= SCAN_FOLD_MAX_BLOCKS) {
+    k_scan_add_fold<<<nblk, SCAN_T, 0, s>>>(out, bsum, (uint32_t)n, total);
+  } else {
+    k_scan_top<<<1, SCAN_T, 0, s>>>(bsum, nblk, total);
+    k_scan_add<<<blocks_for(n, 256), 256, 0, s>>>(out, bsum, (uint32_t)n);
+  }
+}
+
 // digits + single-pass counting sort of (bucket, point) pairs by bucket (the key spaces
 // above the two-pass sort's 2^22)
 static void msm_sort_single(const uint32_t* const* d_scalars, const uint32_t* const* d_points, const uint64_t* ns,
                             const MsmPlan& p, hipStream_t s, int group, uint32_t* keys, uint32_t* vals,
                             uint32_t* ranks, uint32_t* counts, uint32_t* offsets, uint32_t* bsum, uint2* sorted,
                             uint32_t* fix, uint64_t fix_words, uint32_t* scal, uint32_t* hist) {
-  uint32_t nblk = msm_scan_blocks(p.nb);
   const int fixed = p.fixed ? 1 : 0;
   {
     ScopedKTime kt(group == 1 ? "msm_g1_digits" : "msm_g2_digits", s);
-    k_zero_u32<<<blocks_for(p.nb + 1, 256), 256, 0, s>>>(counts, (uint64_t)p.nb + 1);
-    k_zero_u32<<<blocks_for(fix_words, 256), 256, 0, s>>>(fix, fix_words);
+    k_zero2_u32<<<blocks_for((uint64_t)p.nb + 1 + fix_words, 256), 256, 0, s>>>(counts, (uint64_t)p.nb + 1, fix,
+                                                                                 fix_words);
     uint64_t off = 0, soff = 0, hoff = 0;
     for (uint32_t i = 0; i < p.M; i++) {
       const uint64_t n = ns[i];
@@ -855,9 +898,7 @@ static void msm_sort_single(const uint32_t* const* d_scalars, const uint32_t* co
   }
   {
     ScopedKTime kt(group == 1 ? "msm_g1_sort" : "msm_g2_sort", s);
-    k_scan_local<<<nblk, SCAN_T, 0, s>>>(counts, offsets, bsum, p.nb);
-    k_scan_top<<<1, SCAN_T, 0, s>>>(bsum, nblk, offsets + p.nb);
-    k_scan_add<<<blocks_for(p.nb, 256), 256, 0, s>>>(offsets, bsum, p.nb);
+    scan_excl(counts, offsets, p.nb, bsum, offsets + p.nb, s);
     uint64_t off = 0, hoff = 0;
     for (uint32_t i = 0; i < p.M; i++) {
       const uint64_t n = ns[i];
@@ -875,13 +916,6 @@ static void msm_sort_single(const uint32_t* const* d_scalars, const uint32_t* co
     }
     BG_HIP(hipGetLastError());
   }
-}
-
-static void scan_excl(uint32_t* data, uint64_t n, uint32_t* bsum, uint32_t* total, hipStream_t s) {
-  const uint32_t nblk = msm_scan_blocks((uint32_t)n);
-  k_scan_local<<<nblk, SCAN_T, 0, s>>>(data, data, bsum, (uint32_t)n);
-  k_scan_top<<<1, SCAN_T, 0, s>>>(bsum, nblk, total);
-  k_scan_add<<<blocks_for(n, 256), 256, 0, s>>>(data, bsum, (uint32_t)n);
 }
 
 // Key bits of an MSM batch and the split between the two passes (hb low bits in pass 2);
@@ -952,7 +986,7 @@ void msm_sort(const uint32_t* const* d_scalars, const uint32_t* const* d_points,
   }
   {
     ScopedKTime kt(group == 1 ? "msm_g1_sort" : "msm_g2_sort", s);
-    scan_excl(hist1, h1n, bsum, total, s);
+    scan_excl(hist1, hist1, h1n, bsum, total, s);
     uint64_t soff = 0;
     uint32_t blk = 0;
     for (uint32_t i = 0; i < p.M; i++) {
@@ -965,9 +999,10 @@ void msm_sort(const uint32_t* const* d_scalars, const uint32_t* const* d_points,
       soff += 8 * n;
     }
     k_sort_tiles<<<1, 1024, 0, s>>>(hist1, nb1, nbins1, total, seg, ntiles, wbase);
-    k_zero_u32<<<blocks_for(h2n, 256), 256, 0, s>>>(hist2, h2n);
+    // every tile writes all its bins' counts, so hist2 needs no clearing: past the last
+    // tile's counts the scan reads stale words that no later read depends on
     k_sort_p2_count<<<g2, SORT_T, 0, s>>>(tmp, seg, ntiles, wbase, nbins1, hb, hist2);
-    scan_excl(hist2, h2n, bsum, total + 1, s);
+    scan_excl(hist2, hist2, h2n, bsum, total + 1, s);
     k_sort_p2_scatter<<<g2, SORT_T, 0, s>>>(tmp, seg, ntiles, wbase, nbins1, hb, hist2, sorted);
     k_sort_offsets<<<blocks_for((uint64_t)p.nb + 1, 256), 256, 0, s>>>(seg, ntiles, wbase, hist2, p.nb, hb, offsets);
     BG_HIP(hipGetLastError());
