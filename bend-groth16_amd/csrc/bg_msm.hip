@@ -794,20 +794,30 @@ static constexpr uint32_t SET_PTRS_MAX = 8;
 struct PtrTable {
   uint64_t p[SET_PTRS_MAX];
 };
-__global__ void k_set_ptrs(uint64_t* __restrict__ out, PtrTable t, uint32_t M) {
-  if (threadIdx.x < M) out[threadIdx.x] = t.p[threadIdx.x];
+// the pointer table, and the two word ranges an MSM starts from zero (in one launch)
+__global__ void k_set_ptrs(uint64_t* __restrict__ out, PtrTable t, uint32_t M, uint32_t* __restrict__ z1,
+                           uint64_t n1, uint32_t* __restrict__ z2, uint64_t n2) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < M) out[i] = t.p[i];
+  if (i < n1) z1[i] = 0;
+  else if (i - n1 < n2) z2[i - n1] = 0;
 }
 
-void launch_set_ptrs(uint64_t* d_out, const uint64_t* h_ptrs, uint32_t M, hipStream_t s) {
+void launch_set_ptrs(uint64_t* d_out, const uint64_t* h_ptrs, uint32_t M, hipStream_t s, uint32_t* z1, uint64_t n1,
+                     uint32_t* z2, uint64_t n2) {
   if (M <= SET_PTRS_MAX) {
     PtrTable t{};
     for (uint32_t i = 0; i < M; i++) t.p[i] = h_ptrs[i];
-    k_set_ptrs<<<1, 64, 0, s>>>(d_out, t, M);
+    k_set_ptrs<<<blocks_for(std::max<uint64_t>(M, n1 + n2), 256), 256, 0, s>>>(d_out, t, M, z1, n1, z2, n2);
     BG_HIP(hipGetLastError());
   } else {
     BG_REQUIRE(!stream_capturing(s), BG_ERR_INTERNAL, "msm: a captured batch holds at most %u base arrays",
                SET_PTRS_MAX);
     BG_HIP(hipMemcpyAsync(d_out, h_ptrs, 8ull * M, hipMemcpyHostToDevice, s));
+    if (n1 + n2) {
+      k_zero2_u32<<<blocks_for(n1 + n2, 256), 256, 0, s>>>(z1, n1, z2, n2);
+      BG_HIP(hipGetLastError());
+    }
   }
 }
 
@@ -870,8 +880,6 @@ static void msm_sort_single(const uint32_t* const* d_scalars, const uint32_t* co
   const int fixed = p.fixed ? 1 : 0;
   {
     ScopedKTime kt(group == 1 ? "msm_g1_digits" : "msm_g2_digits", s);
-    k_zero2_u32<<<blocks_for((uint64_t)p.nb + 1 + fix_words, 256), 256, 0, s>>>(counts, (uint64_t)p.nb + 1, fix,
-                                                                                 fix_words);
     uint64_t off = 0, soff = 0, hoff = 0;
     for (uint32_t i = 0; i < p.M; i++) {
       const uint64_t n = ns[i];
@@ -968,7 +976,6 @@ void msm_sort(const uint32_t* const* d_scalars, const uint32_t* const* d_points,
   const size_t lds1 = (size_t)nbins1 * 4;
   {
     ScopedKTime kt(group == 1 ? "msm_g1_digits" : "msm_g2_digits", s);
-    k_zero_u32<<<blocks_for(fix_words, 256), 256, 0, s>>>(fix, fix_words);
     uint64_t soff = 0;
     uint32_t blk = 0;
     for (uint32_t i = 0; i < p.M; i++) {

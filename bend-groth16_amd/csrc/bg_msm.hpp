@@ -134,15 +134,19 @@ void launch_unpack16(const uint32_t* in32, uint32_t* out16, uint64_t in_words, h
 // pipeline pieces shared by the per-group translation units
 uint32_t msm_scan_blocks(uint32_t n);
 void launch_zero_u32(uint32_t* p, uint64_t n, hipStream_t s);
-// d_out[i] = h_ptrs[i] for i < M on stream s (the MSM's table of base-array addresses;
-// h_ptrs may be reused once this returns).
-void launch_set_ptrs(uint64_t* d_out, const uint64_t* h_ptrs, uint32_t M, hipStream_t s);
+// d_out[i] = h_ptrs[i] for i < M on stream s (the MSM's table of base-array addresses,
+// passed as kernel arguments: nothing on the host to copy from in a recorded sequence;
+// h_ptrs may be reused once this returns), and z1[0, n1), z2[0, n2) cleared in the same
+// launch.
+void launch_set_ptrs(uint64_t* d_out, const uint64_t* h_ptrs, uint32_t M, hipStream_t s, uint32_t* z1, uint64_t n1,
+                     uint32_t* z2, uint64_t n2);
 // dst[i] = src[i] for i < n (dst in coherent pinned host memory), then *flag = 1 once the
 // copy is complete and visible to the host.
 void launch_publish(const uint32_t* src, uint64_t n, uint32_t* dst, uint32_t* flag, hipStream_t s);
 // digits and the sort of the batch's (bucket, point) entries by bucket: sorted entries,
 // offsets[0 .. nb] (offsets[nb] = entries); the work arrays come from w (offsets lies in
-// w.counts, after nb + 1 words the single-pass sort uses for its counts)
+// w.counts, after nb + 1 words the single-pass sort uses for its counts).  Those counts
+// and fix[0, fix_words) must be zero when it starts (launch_set_ptrs clears them).
 void msm_sort(const uint32_t* const* d_scalars, const uint32_t* const* d_points, const uint64_t* ns,
               const MsmPlan& p, hipStream_t s, int group, MsmWork& w, uint32_t* offsets, uint2* sorted,
               uint32_t* fix, uint64_t fix_words);

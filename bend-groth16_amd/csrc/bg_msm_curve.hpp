@@ -980,7 +980,8 @@ void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* 
     // empty MSM: every bucket empty -> zero outputs
     launch_zero_u32(outPQ, 2 * nout * xw, s);
   } else {
-    launch_set_ptrs(dptrs, hptrs, M, s);
+    // the sorts start from cleared bucket counts and fix-up flags (count | per-bucket flags)
+    launch_set_ptrs(dptrs, hptrs, M, s, counts, (uint64_t)p.nb + 1, fix, 1 + (uint64_t)p.nb);
     msm_sort(d_scalars, d_points, ns, p, s, group, w, offsets, sorted, fix, 1 + (uint64_t)p.nb);
     {
       {
