@@ -75,27 +75,12 @@ __global__ void k_gather(const uint32_t* __restrict__ in, uint32_t* __restrict__
 // of the per-stage table tw.  scale (optional) multiplies each element by scale[index] as
 // it is loaded.
 // src and dst may alias (in-place pass).
-template <bool DIT, int TILE, bool R4>
-__global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass(const uint32_t* src, uint32_t* dst, int log2, int k0, int t,
-                                                          int logC, const uint32_t* __restrict__ tw,
-                                                          const uint32_t* __restrict__ scale) {
-  __shared__ Fp sh[1 << TILE];
-  src += 8 * ((uint64_t)blockIdx.y << log2);
-  dst += 8 * ((uint64_t)blockIdx.y << log2);
-  const int S_log = log2 - k0 - t;
+// The stages of one pass over the tile in LDS (cnt elements: rows of C = 2^logC columns),
+// each round ending with a barrier.
+template <bool DIT, bool R4>
+__device__ __forceinline__ void ntt_stages(Fp* sh, int log2, int k0, int t, int logC, int S_log, uint64_t q0,
+                                           uint32_t cnt, const uint32_t* __restrict__ tw) {
   const uint32_t C = 1u << logC;
-  const uint32_t qgroups = 1u << (S_log - logC);
-  const uint64_t blk = blockIdx.x >> (S_log - logC);
-  const uint64_t q0 = (uint64_t)(blockIdx.x & (qgroups - 1)) << logC;
-  const uint64_t base = (blk << (log2 - k0)) + q0;
-  const uint32_t cnt = 1u << (t + logC);
-  for (uint32_t e = threadIdx.x; e < cnt; e += NTT_THREADS) {
-    uint64_t idx = base + ((uint64_t)(e >> logC) << S_log) + (e & (C - 1));
-    Fp x = load_fp(src + 8 * idx);
-    if (scale) x = Fr::mul(x, load_fp(scale + 8 * idx));
-    sh[e] = x;
-  }
-  __syncthreads();
   // twiddle of stage k for local row residue jl: stage k's table holds omega^(j 2^k)
   // contiguously from n - n/2^k, so the lanes of a row (consecutive c) read consecutive
   // twiddles
@@ -166,6 +151,67 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass(const uint32_t* src, u
     }
     __syncthreads();
   }
+}
+
+template <bool DIT, int TILE, bool R4>
+__global__ __launch_bounds__(NTT_THREADS) void k_ntt_pass(const uint32_t* src, uint32_t* dst, int log2, int k0, int t,
+                                                          int logC, const uint32_t* __restrict__ tw,
+                                                          const uint32_t* __restrict__ scale) {
+  __shared__ Fp sh[1 << TILE];
+  src += 8 * ((uint64_t)blockIdx.y << log2);
+  dst += 8 * ((uint64_t)blockIdx.y << log2);
+  const int S_log = log2 - k0 - t;
+  const uint32_t C = 1u << logC;
+  const uint32_t qgroups = 1u << (S_log - logC);
+  const uint64_t blk = blockIdx.x >> (S_log - logC);
+  const uint64_t q0 = (uint64_t)(blockIdx.x & (qgroups - 1)) << logC;
+  const uint64_t base = (blk << (log2 - k0)) + q0;
+  const uint32_t cnt = 1u << (t + logC);
+  for (uint32_t e = threadIdx.x; e < cnt; e += NTT_THREADS) {
+    uint64_t idx = base + ((uint64_t)(e >> logC) << S_log) + (e & (C - 1));
+    Fp x = load_fp(src + 8 * idx);
+    if (scale) x = Fr::mul(x, load_fp(scale + 8 * idx));
+    sh[e] = x;
+  }
+  __syncthreads();
+  ntt_stages<DIT, R4>(sh, log2, k0, t, logC, S_log, q0, cnt, tw);
+  for (uint32_t e = threadIdx.x; e < cnt; e += NTT_THREADS) {
+    uint64_t idx = base + ((uint64_t)(e >> logC) << S_log) + (e & (C - 1));
+    store_fp(dst + 8 * idx, sh[e]);
+  }
+}
+
+// The last pass of a DIF transform and the first of the DIT transform that follows it
+// (the same stages over the same tiles) in one: DIF stages with tw_dif, each element times
+// scale[index] (the DIT pass's scale on load), DIT stages with tw_dit; one load and one
+// store of the tile instead of two each.
+template <int TILE, bool R4>
+__global__ __launch_bounds__(NTT_THREADS) void k_ntt_mid(const uint32_t* src, uint32_t* dst, int log2, int k0, int t,
+                                                         int logC, const uint32_t* __restrict__ tw_dif,
+                                                         const uint32_t* __restrict__ scale,
+                                                         const uint32_t* __restrict__ tw_dit) {
+  __shared__ Fp sh[1 << TILE];
+  src += 8 * ((uint64_t)blockIdx.y << log2);
+  dst += 8 * ((uint64_t)blockIdx.y << log2);
+  const int S_log = log2 - k0 - t;
+  const uint32_t C = 1u << logC;
+  const uint32_t qgroups = 1u << (S_log - logC);
+  const uint64_t blk = blockIdx.x >> (S_log - logC);
+  const uint64_t q0 = (uint64_t)(blockIdx.x & (qgroups - 1)) << logC;
+  const uint64_t base = (blk << (log2 - k0)) + q0;
+  const uint32_t cnt = 1u << (t + logC);
+  for (uint32_t e = threadIdx.x; e < cnt; e += NTT_THREADS) {
+    uint64_t idx = base + ((uint64_t)(e >> logC) << S_log) + (e & (C - 1));
+    sh[e] = load_fp(src + 8 * idx);
+  }
+  __syncthreads();
+  ntt_stages<false, R4>(sh, log2, k0, t, logC, S_log, q0, cnt, tw_dif);
+  for (uint32_t e = threadIdx.x; e < cnt; e += NTT_THREADS) {
+    uint64_t idx = base + ((uint64_t)(e >> logC) << S_log) + (e & (C - 1));
+    sh[e] = Fr::mul(sh[e], load_fp(scale + 8 * idx));
+  }
+  __syncthreads();
+  ntt_stages<true, R4>(sh, log2, k0, t, logC, S_log, q0, cnt, tw_dit);
   for (uint32_t e = threadIdx.x; e < cnt; e += NTT_THREADS) {
     uint64_t idx = base + ((uint64_t)(e >> logC) << S_log) + (e & (C - 1));
     store_fp(dst + 8 * idx, sh[e]);
@@ -350,6 +396,35 @@ static void launch_pass(bool dit, const uint32_t* src, uint32_t* dst, int log2, 
   BG_HIP(hipGetLastError());
 }
 
+template <int TILE>
+static void launch_mid_t(bool r4, dim3 wgs, uint32_t* v, int log2, const NttPass& p, const uint32_t* tw_dif,
+                         const uint32_t* scale, const uint32_t* tw_dit, hipStream_t s) {
+  if (r4)
+    k_ntt_mid<TILE, true><<<wgs, NTT_THREADS, 0, s>>>(v, v, log2, p.k0, p.t, p.logC, tw_dif, scale, tw_dit);
+  else
+    k_ntt_mid<TILE, false><<<wgs, NTT_THREADS, 0, s>>>(v, v, log2, p.k0, p.t, p.logC, tw_dif, scale, tw_dit);
+}
+
+// In place: a DIF transform with tw_dif (natural -> bit-reversed order), each element times
+// scale[index], a DIT transform with tw_dit (-> natural order); the DIF's last pass and the
+// DIT's first run as one kernel.
+static void dif_scale_dit(uint32_t* v, int log2, const uint32_t* tw_dif, const uint32_t* scale,
+                          const uint32_t* tw_dit, hipStream_t s, unsigned batch) {
+  const NttPlan plan = ntt_plan(log2);
+  for (int i = 0; i + 1 < plan.count; i++)
+    launch_pass(false, v, v, log2, plan, plan.p[i], tw_dif, nullptr, s, batch);
+  const NttPass& m = plan.p[plan.count - 1];
+  const dim3 wgs(1u << (log2 - m.t - m.logC), batch);
+  switch (plan.tile) {
+    case 9: launch_mid_t<9>(plan.r4, wgs, v, log2, m, tw_dif, scale, tw_dit, s); break;
+    case 11: launch_mid_t<11>(plan.r4, wgs, v, log2, m, tw_dif, scale, tw_dit, s); break;
+    default: launch_mid_t<10>(plan.r4, wgs, v, log2, m, tw_dif, scale, tw_dit, s); break;
+  }
+  BG_HIP(hipGetLastError());
+  for (int i = plan.count - 2; i >= 0; i--)
+    launch_pass(true, v, v, log2, plan, plan.p[i], tw_dit, nullptr, s, batch);
+}
+
 // natural order in -> bit-reversed order out; scale multiplies the (natural) input
 static void dif(const uint32_t* src, uint32_t* dst, int log2, const uint32_t* tw, const uint32_t* scale,
                 hipStream_t s, unsigned batch = 1) {
@@ -401,10 +476,17 @@ static void qap_h_core(uint32_t* a, uint32_t* b, uint32_t* c, uint32_t* h, int l
                        hipStream_t s, unsigned batch) {
   const uint64_t n = 1ull << log2;
   uint32_t* v3[3] = {a, b, c};
+  // evaluations -> coefficients (bit-reversed, times n) -> n^-1 g^k -> coset evaluations.
+  // Domains up to 2^17 fuse the middle passes (K = 10 .. 16 proofs 1-2 % faster); the 2^19
+  // domain keeps them apart: fused, the 2^18 proof's kernels measured 1 % less serialised
+  // but the proof 0.5 % slower overlapped (its longer blocks wait for SIMDs beside the MSMs)
   for (uint32_t* v : v3) {
-    // coefficients (bit-reversed, times n) -> n^-1 g^k scale on load -> coset evaluations
-    dif(v, v, log2, T.tw_inv, nullptr, s, batch);
-    dit(v, v, log2, T.tw_fwd, T.ninv_g_rev, s, batch);
+    if (log2 <= 17) {
+      dif_scale_dit(v, log2, T.tw_inv, T.ninv_g_rev, T.tw_fwd, s, batch);
+    } else {
+      dif(v, v, log2, T.tw_inv, nullptr, s, batch);
+      dit(v, v, log2, T.tw_fwd, T.ninv_g_rev, s, batch);
+    }
   }
   k_qap_point<<<nblk(n * batch, 256), 256, 0, s>>>(a, b, c, n * batch, to_fp(T.zinv));
   BG_HIP(hipGetLastError());
