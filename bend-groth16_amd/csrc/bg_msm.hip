@@ -209,7 +209,6 @@ uint64_t msm_work_bytes(const MsmPlan& p, int group) {
                + 4 * (2 + ngroups + ngroups / p.K)  // reduction exception lists
                + (uint64_t)p.nb * xb             // buckets
                + 2ull * p.acc_threads * (8 + xb)  // partial slots
-               + (uint64_t)p.acc_threads * (group == 1 ? 152 : 296)  // staged digit-form runs
                + 4 * mcap * (8 + xb)             // merge levels
                + 2 * ngroups * xb + 2 * nout * xb + (1u << 16)
                + 32 * p.n                         // reduced scalars

@@ -49,7 +49,7 @@ int fixed_window_bits(const uint64_t* ns, uint32_t M, int group);
 // Work buffers of one in-flight MSM (reused across calls).
 struct MsmWork {
   DevBuf keys, vals, ranks, counts, sorted, buckets, pkeys, pflags, ppts, mkeys, mflags, mpts, lvlS, lvlT, outPQ,
-      scan_tmp, fix, rfix, ptrs, scal, hist, mq, stage;
+      scan_tmp, fix, rfix, ptrs, scal, hist, mq;
   HostBuf host_out, host_ptrs;
   // Coherent pinned landing of the outputs of a recorded MSM (no event nodes in a graph):
   // PUB_HEAD words whose first is a flag the device raises once the outputs follow it.
@@ -71,10 +71,10 @@ struct MsmWork {
   // launch sequences all walk this one list
   template <class Self>
   static auto buf_list(Self& w) {
-    return std::array<decltype(&w.keys), 23>{&w.keys, &w.vals, &w.ranks, &w.counts, &w.sorted, &w.buckets,
+    return std::array<decltype(&w.keys), 22>{&w.keys, &w.vals, &w.ranks, &w.counts, &w.sorted, &w.buckets,
                                              &w.pkeys, &w.pflags, &w.ppts, &w.mkeys, &w.mflags, &w.mpts,
                                              &w.lvlS, &w.lvlT, &w.outPQ, &w.scan_tmp, &w.fix, &w.rfix,
-                                             &w.ptrs, &w.scal, &w.hist, &w.mq, &w.stage};
+                                             &w.ptrs, &w.scal, &w.hist, &w.mq};
   }
   auto bufs() const { return buf_list(*this); }
   auto bufs() { return buf_list(*this); }

@@ -103,13 +103,13 @@ __device__ __forceinline__ void store_acc(uint32_t* __restrict__ p, const typena
   store_vec<A::xyzz_words>(p, w);
 }
 
-// A closed run's digit-form sum waits in a per-thread staging record until a wave-wide
-// write-out converts it to the word domain (the conversion then runs once for every lane
-// with a waiting run, instead of once per step at which any lane of the wave closes one).
-// Record of thread t, coordinate k (the k-th digit vector of A::Pt): digits 0..3 and 4..7
-// as 16-byte chunks 2k and 2k + 1 of column t, digit 8 in a word array after the chunks,
-// then the run's key and destination kind (a word pair), so a wave's lanes write and read
-// whole contiguous spans and nothing of the waiting run is held in registers.
+// A closed run's digit-form sum waits in a per-thread staging record (in LDS) until a
+// wave-wide write-out converts it to the word domain (the conversion then runs once for
+// every lane with a waiting run, instead of once per step at which any lane of the wave
+// closes one).  Record of thread t, coordinate k (the k-th digit vector of A::Pt): digits
+// 0..3 and 4..7 as 16-byte chunks 2k and 2k + 1 of column t, digit 8 in a word array after
+// the chunks, then the run's key and destination kind (a word pair), so a wave's lanes
+// touch contiguous spans and nothing of the waiting run is held in registers.
 template <class A>
 struct StageOf {
   static constexpr uint32_t coords = sizeof(typename A::Pt) / sizeof(fd::Fd);
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(256, AccWaves<F>::value) void k_accumulate(const ui
                                                     uint32_t nthreads, uint32_t* __restrict__ buckets,
                                                     uint32_t* __restrict__ pkeys, uint32_t* __restrict__ pflags,
                                                     uint32_t* __restrict__ ppts, uint32_t nb,
-                                                    uint32_t* __restrict__ fix, uint32_t* __restrict__ stage) {
+                                                    uint32_t* __restrict__ fix) {
   using C = Curve<F>;
   using A = typename AccOf<F>::type;
   static_assert(A::aff_words == C::aff_words && A::xyzz_words == C::xyzz_words, "point layout");
@@ -209,6 +209,9 @@ __global__ __launch_bounds__(256, AccWaves<F>::value) void k_accumulate(const ui
   // complete bucket; SK_HEAD: head slot 2t, flag 1; SK_TAIL | flag: tail slot 2t + 1)
   constexpr uint32_t SK_BUCKET = 1, SK_HEAD = 2, SK_TAIL = 4;
   bool staged = false;
+  // the block's staging records live in LDS (column threadIdx.x of 256)
+  __shared__ uint32_t stage_lds[PF ? (uint32_t)(StageOf<A>::bytes_per_thread / 4) * 256 : 1];
+  const uint32_t lt = threadIdx.x;
   // Step e == end closes the last run; step end + 1 writes out what is still staged.
   // (G2 converts and writes each run where it closes: its addition is 2.8 times G1's, so
   // the conversion is a smaller share of a step; staged, its register file fits only with
@@ -221,7 +224,7 @@ __global__ __launch_bounds__(256, AccWaves<F>::value) void k_accumulate(const ui
     // a lane about to stage a run while one waits: the wave writes its staged runs out
     if (STAGE && __any((closes || drain) && staged)) {
       if (staged) {
-        const uint2 meta = stage_meta<A>(stage, nthreads, t);  // key, kind
+        const uint2 meta = stage_meta<A>(stage_lds, 256, lt);  // key, kind
         uint32_t* dst;
         if (meta.y == SK_BUCKET) {
           dst = buckets + (uint64_t)A::xyzz_words * meta.x;
@@ -231,7 +234,7 @@ __global__ __launch_bounds__(256, AccWaves<F>::value) void k_accumulate(const ui
           pflags[slot] = meta.y == SK_HEAD ? 1u : meta.y & 1u;
           dst = ppts + (uint64_t)A::xyzz_words * slot;
         }
-        stage_write<A>(stage, nthreads, t, dst);
+        stage_write<A>(stage_lds, 256, lt, dst);
         staged = false;
       }
     }
@@ -243,7 +246,7 @@ __global__ __launch_bounds__(256, AccWaves<F>::value) void k_accumulate(const ui
       if (bad) flag_bucket<F>(cur, nb, fix);
       const uint32_t slot = !cp && !cn ? NOKEY_C : (cn ? 2 * t + 1 : 2 * t);
       if constexpr (STAGE) {
-        stage_put<A>(stage, nthreads, t, acc, cur,
+        stage_put<A>(stage_lds, 256, lt, acc, cur,
                      slot == NOKEY_C ? SK_BUCKET : (cn ? SK_TAIL | (cp ? 1u : 0u) : SK_HEAD));
         staged = true;
       } else {
@@ -958,8 +961,6 @@ void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* 
   uint32_t* pkeys = w.pkeys.as<uint32_t>(2ull * p.acc_threads);
   uint32_t* pflags = w.pflags.as<uint32_t>(2ull * p.acc_threads);
   uint32_t* ppts = w.ppts.as<uint32_t>(2ull * p.acc_threads * xw);
-  uint32_t* stage = static_cast<uint32_t*>(
-      w.stage.get((uint64_t)p.acc_threads * StageOf<typename AccOf<F>::type>::bytes_per_thread));
   // merge levels ping-pong between two halves of mcap items each (sized for the narrower
   // of the two merge kernels' blocks)
   const uint64_t mcap = blocks_of(p.acc_threads, std::min<uint32_t>(MergeT<F>::value, MERGE_QT));
@@ -989,7 +990,7 @@ void enqueue_curve(uint32_t M, const uint32_t* const* d_points, const uint32_t* 
         ScopedKTime kt(group == 1 ? "msm_g1_accumulate" : "msm_g2_accumulate", s);
         k_accumulate<F><<<blocks_of(p.acc_threads, 256), 256, 0, s>>>(dptrs, p.kspan, sorted, offsets + p.nb, p.L,
                                                                        p.acc_threads, buckets, pkeys, pflags, ppts,
-                                                                       p.nb, fix, stage);
+                                                                       p.nb, fix);
       }
       ScopedKTime kt(group == 1 ? "msm_g1_merge" : "msm_g2_merge", s);
       {
