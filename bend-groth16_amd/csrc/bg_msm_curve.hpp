@@ -96,12 +96,6 @@ __device__ __forceinline__ typename A::Raw load_point(const uint32_t* __restrict
   load_vec<A::aff_words>(p, w);
   return A::load(w);
 }
-template <class A>
-__device__ __forceinline__ void store_acc(uint32_t* __restrict__ p, const typename A::Pt& acc) {
-  uint32_t w[A::xyzz_words];
-  A::store(w, acc);
-  store_vec<A::xyzz_words>(p, w);
-}
 
 // A closed run's digit-form sum waits in a per-thread staging record (in LDS) until a
 // wave-wide write-out converts it to the word domain (the conversion then runs once for
@@ -209,20 +203,17 @@ __global__ __launch_bounds__(256, AccWaves<F>::value) void k_accumulate(const ui
   // complete bucket; SK_HEAD: head slot 2t, flag 1; SK_TAIL | flag: tail slot 2t + 1)
   constexpr uint32_t SK_BUCKET = 1, SK_HEAD = 2, SK_TAIL = 4;
   bool staged = false;
-  // the block's staging records live in LDS (column threadIdx.x of 256)
-  __shared__ uint32_t stage_lds[PF ? (uint32_t)(StageOf<A>::bytes_per_thread / 4) * 256 : 1];
+  // the block's staging records live in LDS (column threadIdx.x of 256): 38 KB for G1, 74 KB
+  // for G2 (two G2 blocks per CU, as its registers allow)
+  __shared__ uint32_t stage_lds[(uint32_t)(StageOf<A>::bytes_per_thread / 4) * 256];
   const uint32_t lt = threadIdx.x;
   // Step e == end closes the last run; step end + 1 writes out what is still staged.
-  // (G2 converts and writes each run where it closes: its addition is 2.8 times G1's, so
-  // the conversion is a smaller share of a step; staged, its register file fits only with
-  // the MSM base looked up at every step, and its proofs measured no faster.)
-  constexpr bool STAGE = PF;
   for (uint32_t e = start;; e++) {
     const bool drain = e > end;
     const bool last = e == end;
     const bool closes = !drain && e > start && ent.x != cur;
     // a lane about to stage a run while one waits: the wave writes its staged runs out
-    if (STAGE && __any((closes || drain) && staged)) {
+    if (__any((closes || drain) && staged)) {
       if (staged) {
         const uint2 meta = stage_meta<A>(stage_lds, 256, lt);  // key, kind
         uint32_t* dst;
@@ -244,28 +235,12 @@ __global__ __launch_bounds__(256, AccWaves<F>::value) void k_accumulate(const ui
       const bool cp = first && prevkey == cur;
       const bool cn = last && nextkey == cur;
       if (bad) flag_bucket<F>(cur, nb, fix);
-      const uint32_t slot = !cp && !cn ? NOKEY_C : (cn ? 2 * t + 1 : 2 * t);
-      if constexpr (STAGE) {
-        stage_put<A>(stage_lds, 256, lt, acc, cur,
-                     slot == NOKEY_C ? SK_BUCKET : (cn ? SK_TAIL | (cp ? 1u : 0u) : SK_HEAD));
-        staged = true;
-      } else {
-        uint32_t* dst;
-        if (slot == NOKEY_C) {
-          dst = buckets + (uint64_t)A::xyzz_words * cur;
-        } else {
-          pkeys[slot] = cur;
-          pflags[slot] = cp ? 1u : 0u;
-          dst = ppts + (uint64_t)A::xyzz_words * slot;
-        }
-        store_acc<A>(dst, acc);
-      }
+      const uint32_t kind = !cp && !cn ? SK_BUCKET : (cn ? SK_TAIL | (cp ? 1u : 0u) : SK_HEAD);
+      stage_put<A>(stage_lds, 256, lt, acc, cur, kind);
+      staged = true;
       first = false;
     }
-    if (last) {
-      if constexpr (STAGE) continue;
-      break;
-    }
+    if (last) continue;
     const bool more = e + 1 < end;
     typename A::Raw pt;
     uint2 nxt;  // entry e + 1
