@@ -55,21 +55,9 @@ static_assert(same_digits(FqP::p29, fd::FqDig::p29) && FqP::n0_29 == fd::FqDig::
                   FqP::p2[7] == fd::FqDig::top2p,
               "digit-domain constants of Fq");
 
-// W words (a multiple of 4, 16-byte aligned) between global memory and registers
-template <int W>
-__device__ __forceinline__ void load_vec(const uint32_t* __restrict__ p, uint32_t* w) {
-  const uint4* q = reinterpret_cast<const uint4*>(p);
-#pragma unroll
-  for (int i = 0; i < W / 4; i++) {
-    const uint4 v = q[i];
-    w[4 * i] = v.x;
-    w[4 * i + 1] = v.y;
-    w[4 * i + 2] = v.z;
-    w[4 * i + 3] = v.w;
-  }
-}
-// The same load through a global-address-space pointer (bases reached through the pointer
-// table are generic pointers, and flat loads make every later wait a wait for all loads).
+// W words (a multiple of 4, 16-byte aligned) between global memory and registers (loads
+// through a global-address-space pointer: bases reached through the pointer table are
+// generic pointers, and flat loads would make every later wait a wait for all loads)
 template <int W>
 __device__ __forceinline__ void load_vec_global(const uint32_t* p, uint32_t* w) {
   typedef uint32_t U4 __attribute__((ext_vector_type(4)));
@@ -89,12 +77,6 @@ __device__ __forceinline__ void store_vec(uint32_t* __restrict__ p, const uint32
   uint4* q = reinterpret_cast<uint4*>(p);
 #pragma unroll
   for (int i = 0; i < W / 4; i++) q[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
-}
-template <class A>
-__device__ __forceinline__ typename A::Raw load_point(const uint32_t* __restrict__ p) {
-  uint32_t w[A::aff_words];
-  load_vec<A::aff_words>(p, w);
-  return A::load(w);
 }
 
 // A closed run's digit-form sum waits in a per-thread staging record (in LDS) until a
@@ -183,18 +165,16 @@ __global__ __launch_bounds__(256, AccWaves<F>::value) void k_accumulate(const ui
   // conversions at a run's start (two reductions into the digit domain) and at its write
   // (four packings into words) are long, so each is emitted once (a smaller kernel for the
   // instruction cache).  Step e == end closes the last run.  Loads run ahead of the
-  // additions: G1 (whose 16-word point fits beside the accumulator within three waves per
-  // SIMD) requests the entry two steps ahead and the next point's words before the current
-  // addition, at addresses kept in bounds on every step (the last entry's index, point 0
-  // of the current MSM), so the loads are unconditional and every wait is for the oldest
-  // loads only; G2 requests the next entry.
-  constexpr bool PF = A::aff_words <= 16;
+  // additions: the entry two steps ahead and the next point's words are requested before
+  // the current addition, at addresses kept in bounds on every step (the last entry's
+  // index, point 0 of the current MSM), so the loads are unconditional and every wait is
+  // for the oldest loads only (G1 within three waves per SIMD, G2 within two).
   uint2 ent = sorted[start];                                                 // entry e
-  uint2 ent1 = start + 1 < end ? sorted[start + 1] : make_uint2(NOKEY_C, 0u);  // entry e + 1 (G1)
+  uint2 ent1 = start + 1 < end ? sorted[start + 1] : make_uint2(NOKEY_C, 0u);  // entry e + 1
   const uint32_t* points = entry_points(ptrs, ent.x, kspan);  // base of entry e's MSM
   uint32_t kend = (ent.x / kspan + 1) * kspan;               // end of that MSM's keys
-  uint32_t w[A::aff_words];
-  if constexpr (PF) load_vec_global<A::aff_words>(points + (uint64_t)A::aff_words * (ent.y & 0x7FFFFFFFu), w);
+  uint32_t w[A::aff_words];                                   // entry e's point
+  load_vec_global<A::aff_words>(points + (uint64_t)A::aff_words * (ent.y & 0x7FFFFFFFu), w);
   uint32_t cur = NOKEY_C;
   bool first = true;
   uint32_t bad = 0;
@@ -241,32 +221,16 @@ __global__ __launch_bounds__(256, AccWaves<F>::value) void k_accumulate(const ui
       first = false;
     }
     if (last) continue;
-    const bool more = e + 1 < end;
-    typename A::Raw pt;
-    uint2 nxt;  // entry e + 1
-    if constexpr (PF) {
-      // entry e + 1's MSM base (rarely a new one) before the entry two ahead is requested
-      // (keys ascend along the slice, so a key at or past kend belongs to a later MSM)
-      if (more && ent1.x >= kend) {
-        points = entry_points(ptrs, ent1.x, kspan);
-        kend = (ent1.x / kspan + 1) * kspan;
-      }
-      uint2 ent2 = sorted[e + 2 < end ? e + 2 : end - 1];
-      if (e + 2 >= end) ent2 = make_uint2(NOKEY_C, 0u);
-      pt = A::load(w);
-      load_vec_global<A::aff_words>(points + (uint64_t)A::aff_words * (ent1.y & 0x7FFFFFFFu), w);
-      nxt = ent1;
-      ent1 = ent2;
-    } else {
-      // G2: the next entry one step ahead, this entry's point loaded here
-      if (ent.x >= kend) {
-        points = entry_points(ptrs, ent.x, kspan);
-        kend = (ent.x / kspan + 1) * kspan;
-      }
-      pt = load_point<A>(points + (uint64_t)A::aff_words * (ent.y & 0x7FFFFFFFu));
-      nxt = sorted[more ? e + 1 : e];
-      if (!more) nxt = make_uint2(NOKEY_C, 0u);
+    // entry e + 1's MSM base (rarely a new one) before the entry two ahead is requested
+    // (keys ascend along the slice, so a key at or past kend belongs to a later MSM)
+    if (e + 1 < end && ent1.x >= kend) {
+      points = entry_points(ptrs, ent1.x, kspan);
+      kend = (ent1.x / kspan + 1) * kspan;
     }
+    uint2 ent2 = sorted[e + 2 < end ? e + 2 : end - 1];
+    if (e + 2 >= end) ent2 = make_uint2(NOKEY_C, 0u);
+    const typename A::Raw pt = A::load(w);
+    load_vec_global<A::aff_words>(points + (uint64_t)A::aff_words * (ent1.y & 0x7FFFFFFFu), w);
     const bool neg = (ent.y >> 31) != 0;
     if (ent.x != cur) {
       cur = ent.x;
@@ -275,7 +239,8 @@ __global__ __launch_bounds__(256, AccWaves<F>::value) void k_accumulate(const ui
     } else {
       acc = A::madd(acc, pt, neg, bad);
     }
-    ent = nxt;
+    ent = ent1;
+    ent1 = ent2;
   }
 }
 
