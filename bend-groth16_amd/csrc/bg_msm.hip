@@ -103,6 +103,12 @@ static void finish_plan(MsmPlan& p, int group) {
   uint64_t L;
   if (lenv) {
     L = lenv;
+  } else if (p.fixed && tuned && tt.L) {
+    // a fixed-base table's tuned slice length is taken as is, a partial device round
+    // included: the 2^18 proof's B2 MSM (3.9 M entries) in 60-entry slices uses half the
+    // device beside the G1 batch on its own stream (3.874 -> 3.779 ms per proof) where the
+    // rounding below would give a full round of 30
+    L = tt.L;
   } else {
     const uint64_t cap = accumulate_threads_per_round(group);
     const uint64_t target = tuned && tt.L ? (uint64_t)tt.L : 40;
